@@ -1,0 +1,455 @@
+/*
+ * sphk_oracle.c -- CPU oracle for the SPH interaction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see sphk_oracle.h).  Scalar FP64, compiled with
+ * -ffp-contract=off -fno-fast-math so that every operation is one IEEE
+ * rounding in the order written.  The operation order inside each pair term
+ * follows the reference's scalar functors line by line (cited per block), so
+ * the oracle reproduces their FP64 results, not merely their mathematics.
+ *
+ * OpenMP parallelises over walks only; per-walk arithmetic is sequential in
+ * j-list order, so results do not depend on the thread count.
+ */
+#include "sphk_oracle.h"
+
+#include <math.h>
+#include <stddef.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define JROW(w, k) (j_idx ? j_idx[(k)] : (k))
+
+/* hdr_kernel.hpp:246-250 -- W(q) = C0 (1-q)_+^4 (1+4q) */
+double orc_kernel0th(double q)
+{
+    double rmin = ((1. - q > 0.) ? 1. - q : 0.);
+    double rmin2 = rmin * rmin;
+    return ORC_CEFF0 * rmin2 * rmin2 * (1. + 4. * q);
+}
+
+/* hdr_kernel.hpp:251-257 -- W'(q) = C1 [(1-q)_+^4 - (1-q)_+^3 (1+4q)] */
+double orc_kernel1st(double q)
+{
+    double rmin = ((1. - q > 0.) ? 1. - q : 0.);
+    double rmin2 = rmin * rmin;
+    double rmin3 = rmin * rmin2;
+    double rmin4 = rmin2 * rmin2;
+    return ORC_CEFF1 * (rmin4 - rmin3 * (1. + 4. * q));
+}
+
+static void set_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+/* Scalar calcDensity, hdr_density.hpp:254-363. */
+void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                 orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
+                 orc_dens_out out, int nthreads)
+{
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int w = 0; w < n_walk; w++) {
+        const int jb = j_off[w], je = j_off[w + 1];
+        for (int i = i_off[w]; i < i_off[w + 1]; i++) {
+            const int64_t id_i = pi.id[i];
+            const double px_i = pi.x[i], py_i = pi.y[i], pz_i = pi.z[i];
+            const double vx_i = pi.vx[i], vy_i = pi.vy[i], vz_i = pi.vz[i];
+            double hs_i = pi.ksr[i];
+
+            /* three fixed-point passes, hdr_density.hpp:264-302 */
+            for (int repeat = 0; repeat < 3; repeat++) {
+                double hi_i = 1. / hs_i;
+                double hi3_i = hi_i * hi_i * hi_i; /* hdr_dimension.hpp:121-123 */
+                double rh_i = 0.;
+                int64_t nj_i = 0;
+                for (int k = jb; k < je; k++) {
+                    const int j = JROW(w, k);
+                    double dx = px_i - pj.x[j];
+                    double dy = py_i - pj.y[j];
+                    double dz = pz_i - pj.z[j];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    double r1 = sqrt(r2);
+                    double q = r1 * hi_i;
+                    double kw0 = orc_kernel0th(q);
+                    double rhj = pj.m[j] * hi3_i * kw0;
+                    rh_i += rhj;
+                    nj_i += ((q < 1.) ? 1 : 0);
+                }
+                /* hdr_density.hpp:288-299; pow(m/rho,1/3) is hdr_dimension.hpp:127-130 */
+                double buf_hs = eta * ksrh * pow(pi.m[i] / rh_i, 1. / 3.);
+                buf_hs = ((buf_hs < hmax) ? buf_hs : hmax);
+                out.dens[i] = rh_i;
+                out.np[i] = nj_i;
+                out.ksr[i] = buf_hs;
+                out.itr[i] = (buf_hs > pi.rs[i]) ? 1 : 0;
+                hs_i = buf_hs;
+            }
+
+            /* grad-h / divergence / curl pass, hdr_density.hpp:304-349 */
+            double hi_i = 1. / hs_i;
+            double hi4_i = hi_i * (hi_i * hi_i * hi_i);
+            double grdh_i = 0., divv_i = 0., rotx_i = 0., roty_i = 0., rotz_i = 0.;
+            for (int k = jb; k < je; k++) {
+                const int j = JROW(w, k);
+                double dpx = px_i - pj.x[j];
+                double dpy = py_i - pj.y[j];
+                double dpz = pz_i - pj.z[j];
+                double dvx = vx_i - pj.vx[j];
+                double dvy = vy_i - pj.vy[j];
+                double dvz = vz_i - pj.vz[j];
+                double r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+                double ri = 1. / sqrt(r2);
+                ri = ((id_i != (int64_t)pj.id[j]) ? ri : 0.);
+                double r1 = r2 * ri;
+                double q = r1 * hi_i;
+                double kw0 = orc_kernel0th(q);
+                double kw1 = orc_kernel1st(q);
+                double m_j = pj.m[j];
+                double ghj = 3. * kw0;
+                ghj += q * kw1;
+                grdh_i -= ghj * hi4_i * m_j;
+                double dw = m_j * hi4_i * kw1 * ri;
+                double dwx = dw * dpx, dwy = dw * dpy, dwz = dw * dpz;
+                divv_i -= dvx * dwx;
+                divv_i -= dvy * dwy;
+                divv_i -= dvz * dwz;
+                rotx_i += dvy * dwz;
+                roty_i += dvz * dwx;
+                rotz_i += dvx * dwy;
+                rotx_i -= dvz * dwy;
+                roty_i -= dvx * dwz;
+                rotz_i -= dvy * dwx;
+            }
+            /* hdr_density.hpp:351-362 (no NaN guard on Omega, as in the CPU functor) */
+            double dens_i = out.dens[i];
+            double deni_i = 1. / dens_i;
+            double omgi_i = 1. / (1. + hs_i * deni_i * grdh_i / 3.);
+            double rot2 = rotx_i * rotx_i + roty_i * roty_i + rotz_i * rotz_i;
+            double rotv_i = rot2 * ((rot2 != 0.) ? 1. / sqrt(rot2) : 0.);
+            rotv_i *= deni_i * omgi_i;
+            divv_i *= deni_i * omgi_i;
+            out.divv[i] = divv_i;
+            out.rotv[i] = rotv_i;
+            out.grdh[i] = omgi_i;
+        }
+    }
+}
+
+/* Scalar calcHydro, hdr_hydro.hpp:320-421. */
+void orc_hydro(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+               orc_hydro_src pi, orc_hydro_src pj, orc_hydro_out out, int nthreads)
+{
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int w = 0; w < n_walk; w++) {
+        const int jb = j_off[w], je = j_off[w + 1];
+        for (int i = i_off[w]; i < i_off[w + 1]; i++) {
+            const int64_t id_i = pi.id[i];
+            const double px_i = pi.x[i], py_i = pi.y[i], pz_i = pi.z[i];
+            const double vx_i = pi.vx[i], vy_i = pi.vy[i], vz_i = pi.vz[i];
+            const double hi_i = pi.hinv[i], pp_i = pi.pres[i], pc_i = pi.presc[i];
+            const double bs_i = pi.bswt[i], dn_i = pi.dens[i], cs_i = pi.vsnd[i];
+            const double al_i = pi.alph[i], au_i = pi.alphu[i], eg_i = pi.thrm[i];
+            const double et_i = pi.eta[i];
+            const double hi4_i = hi_i * (hi_i * hi_i * hi_i);
+            double achx = 0., achy = 0., achz = 0., udot = 0., vsmx = 0., difu = 0.;
+            double g1x = 0., g1y = 0., g1z = 0.;
+            for (int k = jb; k < je; k++) {
+                const int j = JROW(w, k);
+                double dpx = px_i - pj.x[j];
+                double dpy = py_i - pj.y[j];
+                double dpz = pz_i - pj.z[j];
+                double dvx = vx_i - pj.vx[j];
+                double dvy = vy_i - pj.vy[j];
+                double dvz = vz_i - pj.vz[j];
+                double r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+                double ri = ((id_i != (int64_t)pj.id[j]) ? 1. / sqrt(r2) : 0.);
+                double r1 = r2 * ri;
+                double q_i = r1 * hi_i;
+                double q_j = r1 * pj.hinv[j];
+                double hj = pj.hinv[j];
+                double hi4_j = hj * (hj * hj * hj);
+                double dw_i = hi4_i * orc_kernel1st(q_i);
+                double dw_j = hi4_j * orc_kernel1st(q_j);
+                double ka = (dw_i + dw_j) * pj.m[j];
+
+                double rv = dpx * dvx + dpy * dvy + dpz * dvz;
+                double w_ij = rv * ri;
+                double w0 = ((w_ij < 0.) ? w_ij : 0.);
+                double vs = cs_i + pj.vsnd[j] - 3. * w0;
+                double rhi = 1. / (dn_i + pj.dens[j]);
+                double av0 = (bs_i + pj.bswt[j]) * (al_i + pj.alph[j]) * vs * w0;
+
+                vsmx = ((vsmx > vs) ? vsmx : vs);
+
+                double ta = (pc_i + pj.presc[j] - 0.5 * av0 * rhi) * ka * ri;
+                achx -= ta * dpx;
+                achy -= ta * dpy;
+                achz -= ta * dpz;
+
+                double vsu2 = fabs(pp_i - pj.pres[j]) * rhi * 2.;
+                double vsui = ((vsu2 != 0.) ? 1. / sqrt(vsu2) : 0.);
+                double vsu = vsu2 * vsui;
+                double du = eg_i - pj.thrm[j];
+
+                udot += ka * (pc_i * w_ij - rhi * (0.25 * av0 * w0 - (au_i + pj.alphu[j]) * vsu * du));
+                difu += rhi * du * ka * ri;
+
+                double dg = pj.m[j] * ri * (et_i * dw_i + pj.eta[j] * dw_j);
+                g1x += dg * dpx;
+                g1y += dg * dpy;
+                g1z += dg * dpz;
+            }
+            out.ax[i] = achx * 0.5;
+            out.ay[i] = achy * 0.5;
+            out.az[i] = achz * 0.5;
+            out.udot[i] = udot * 0.5;
+            out.vsmx[i] = vsmx;
+            out.diffu[i] = difu * 2.;
+            out.gx[i] = g1x * 0.5;
+            out.gy[i] = g1y * 0.5;
+            out.gz[i] = g1z * 0.5;
+        }
+    }
+}
+
+/* One calcGravity<TJ> call over one list, hdr_gravity.hpp:295-338 (accumulating). */
+static void grav_list(int i, orc_grav_src pi, const int32_t *idx, int kb, int ke,
+                      orc_grav_src pj, orc_grav_out out)
+{
+    const double px_i = pi.x[i], py_i = pi.y[i], pz_i = pi.z[i], e2_i = pi.eps2[i];
+    double ax = 0., ay = 0., az = 0., pt = 0., et = 0.;
+    for (int k = kb; k < ke; k++) {
+        const int j = idx ? idx[k] : k;
+        double dpx = px_i - pj.x[j];
+        double dpy = py_i - pj.y[j];
+        double dpz = pz_i - pj.z[j];
+        double r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+        double re2_i = r2 + e2_i;
+        double rei_i = 1. / sqrt(re2_i);
+        double rei3_i = rei_i * rei_i * rei_i;
+        double re2_j = r2 + pj.eps2[j];
+        double rei_j = 1. / sqrt(re2_j);
+        double rei3_j = rei_j * rei_j * rei_j;
+        double m_j = pj.m[j];
+        double dg2 = m_j * (rei3_i + rei3_j);
+        pt -= m_j * (rei_i + rei_j);
+        ax -= dpx * dg2;
+        ay -= dpy * dg2;
+        az -= dpz * dg2;
+        et += m_j * rei3_i;
+    }
+    ax *= 0.5;
+    ay *= 0.5;
+    az *= 0.5;
+    pt *= 0.5;
+    out.ax[i] += ax;
+    out.ay[i] += ay;
+    out.az[i] += az;
+    out.pot[i] += pt;
+    out.eta[i] += et;
+}
+
+void orc_gravity(int n_walk, const int32_t *i_off,
+                 const int32_t *j_off, const int32_t *j_idx, orc_grav_src pj,
+                 const int32_t *s_off, const int32_t *s_idx, orc_grav_src ps,
+                 orc_grav_src pi, orc_grav_out out, int nthreads)
+{
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int w = 0; w < n_walk; w++) {
+        for (int i = i_off[w]; i < i_off[w + 1]; i++) {
+            out.ax[i] = 0.; out.ay[i] = 0.; out.az[i] = 0.; out.pot[i] = 0.; out.eta[i] = 0.;
+            grav_list(i, pi, j_idx, j_off[w], j_off[w + 1], pj, out);
+            if (s_off) grav_list(i, pi, s_idx, s_off[w], s_off[w + 1], ps, out);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+
+void orc_eos_balsara(int n, orc_state s, orc_params p)
+{
+    for (int i = 0; i < n; i++) {
+        /* hdr_hgas.hpp:339-341 */
+        s.pres[i] = p.ninv * s.dens[i] * s.uene[i];
+        s.vsnd[i] = sqrt((1 + p.ninv) * s.pres[i] / s.dens[i]);
+        /* hdr_sph.hpp:79-81 */
+        s.bswt[i] = fabs(s.divv[i])
+            / (fabs(s.divv[i]) + fabs(s.rotv[i]) + 1e-4 * s.vsnd[i] * p.ksrh / s.ksr[i]);
+    }
+}
+
+void orc_hydro_prep(int n, const double *ksr, const double *pres, const double *grdh,
+                    const double *dens, const double *bswt, const double *uene,
+                    double *hinv, double *presc, double *bswt_half, double *thrm)
+{
+    for (int i = 0; i < n; i++) {
+        /* hdr_hydro.hpp:50-60 with umin = 0 (ideal gas, hdr_hgas.hpp:342) */
+        hinv[i] = 1. / ksr[i];
+        presc[i] = pres[i] * grdh[i] / (dens[i] * dens[i]);
+        bswt_half[i] = 0.5 * bswt[i];
+        double ene = uene[i] - 0.;
+        thrm[i] = (ene > 0.) ? ene : 0.;
+    }
+}
+
+void orc_gravity_writeback(int n, orc_state s, orc_params p,
+                           const double *gax, const double *gay, const double *gaz,
+                           const double *gpot, const double *geta)
+{
+    const double ksrhinv = 1. / p.ksrh;
+    for (int i = 0; i < n; i++) {
+        /* hdr_gravity.hpp:109-112 */
+        s.g2x[i] = p.G * gax[i];
+        s.g2y[i] = p.G * gay[i];
+        s.g2z[i] = p.G * gaz[i];
+        s.eta[i] = s.ksr[i] * s.ksr[i] * ksrhinv * ksrhinv * s.grdh[i] / (3. * s.dens[i]) * geta[i];
+        s.pot[i] = p.G * gpot[i];
+    }
+}
+
+void orc_hydro_writeback_tail(int n, orc_state s, orc_params p,
+                              const double *hax, const double *hay, const double *haz,
+                              const double *hgx, const double *hgy, const double *hgz,
+                              const double *udot, const double *vsmx, const double *diffu)
+{
+    const double ksrhinv = 1. / p.ksrh;
+    for (int i = 0; i < n; i++) {
+        /* hdr_hydro.hpp:21-25 */
+        s.achx[i] = hax[i]; s.achy[i] = hay[i]; s.achz[i] = haz[i];
+        s.g1x[i] = hgx[i] * p.G; s.g1y[i] = hgy[i] * p.G; s.g1z[i] = hgz[i] * p.G;
+        s.udot[i] = udot[i];
+        s.vsmx[i] = vsmx[i];
+        s.diffu[i] = diffu[i];
+        /* hdr_sph.hpp:95-97 */
+        s.ax[i] = s.achx[i] + s.g1x[i] + s.g2x[i];
+        s.ay[i] = s.achy[i] + s.g1y[i] + s.g2y[i];
+        s.az[i] = s.achz[i] + s.g1z[i] + s.g2z[i];
+        /* hdr_hgas.hpp:750-762 (umin = 0 for the ideal gas) */
+        double src = -s.divv[i] * (p.alpha_max - s.alph[i]);
+        src = (src > 0.) ? src : 0.;
+        double tauinv = (0.25 * p.ksrh * s.vsnd[i]) / s.ksr[i];
+        s.adot[i] = -(s.alph[i] - p.alpha_min) * tauinv + src;
+        double uene = s.uene[i] - 0.;
+        uene = (uene > 0.) ? uene : 0.;
+        double lim = p.alphu_max - s.alphu[i];
+        lim = (lim > 0.) ? lim : 0.;
+        double srcu = s.ksr[i] * ksrhinv * fabs(s.diffu[i]) / sqrt(uene + p.eps_u) * lim;
+        s.adotu[i] = -(s.alphu[i] - p.alphu_min) * tauinv + srcu;
+    }
+}
+
+double orc_timestep(int n, orc_state s, orc_params p, double dt_prev, double time, double dt_min)
+{
+    double dtc = p.dt_max;
+    for (int i = 0; i < n; i++) {
+        /* hdr_hgas.hpp:477-488 */
+        double dth = p.c_dt * s.ksr[i] / (s.vsmx[i] * p.ksrh);
+        double dtu = p.dt_max;
+        double dtcv = (s.divv[i] < 0.) ? (-p.c_dt / s.divv[i]) : p.dt_max;
+        double gx = s.g1x[i] + s.g2x[i], gy = s.g1y[i] + s.g2y[i], gz = s.g1z[i] + s.g2z[i];
+        double dtg = p.c_dt * sqrt(s.ksr[i] / sqrt(gx * gx + gy * gy + gz * gz));
+        double a = (dth < dtu) ? dth : dtu;
+        double b = (dtg < dtcv) ? dtg : dtcv;
+        double d = (a < b) ? a : b;
+        if (d < dtc) dtc = d;
+    }
+    /* main.cpp:252-270 */
+    double dt = dt_prev;
+    if (dt > dtc) {
+        while (dt > dtc) {
+            dt *= 0.5;
+            if (dt <= dt_min) return -1.;
+        }
+    } else if (2. * dt <= dtc && 2. * dt <= p.dt_max) {
+        double dt2 = 2. * dt;
+        if (time - (double)(int64_t)(time / dt2) * dt2 == 0.) dt *= 2.;
+    }
+    return dt;
+}
+
+void orc_predict(int n, orc_state s, double dt)
+{
+    for (int i = 0; i < n; i++) {
+        /* hdr_hgas.hpp:646-657 */
+        s.x[i] = s.x[i] + s.vx[i] * dt + 0.5 * s.ax[i] * dt * dt;
+        s.y[i] = s.y[i] + s.vy[i] * dt + 0.5 * s.ay[i] * dt * dt;
+        s.z[i] = s.z[i] + s.vz[i] * dt + 0.5 * s.az[i] * dt * dt;
+        s.vx2[i] = s.vx[i] + 0.5 * s.ax[i] * dt;
+        s.vy2[i] = s.vy[i] + 0.5 * s.ay[i] * dt;
+        s.vz2[i] = s.vz[i] + 0.5 * s.az[i] * dt;
+        s.vx[i] = s.vx[i] + s.ax[i] * dt;
+        s.vy[i] = s.vy[i] + s.ay[i] * dt;
+        s.vz[i] = s.vz[i] + s.az[i] * dt;
+        s.uene2[i] = s.uene[i] + 0.5 * s.udot[i] * dt;
+        s.uene[i] = s.uene[i] + s.udot[i] * dt;
+        s.alph2[i] = s.alph[i] + 0.5 * s.adot[i] * dt;
+        s.alph[i] = s.alph[i] + s.adot[i] * dt;
+        s.alphu2[i] = s.alphu[i] + 0.5 * s.adotu[i] * dt;
+        s.alphu[i] = s.alphu[i] + s.adotu[i] * dt;
+    }
+}
+
+void orc_correct(int n, orc_state s, double dt)
+{
+    for (int i = 0; i < n; i++) {
+        /* hdr_hgas.hpp:659-665 */
+        s.vx[i] = s.vx2[i] + 0.5 * s.ax[i] * dt;
+        s.vy[i] = s.vy2[i] + 0.5 * s.ay[i] * dt;
+        s.vz[i] = s.vz2[i] + 0.5 * s.az[i] * dt;
+        s.uene[i] = s.uene2[i] + 0.5 * s.udot[i] * dt;
+        s.alph[i] = s.alph2[i] + 0.5 * s.adot[i] * dt;
+        s.alphu[i] = s.alphu2[i] + 0.5 * s.adotu[i] * dt;
+    }
+}
+
+double orc_energy(int n, orc_state s)
+{
+    double e = 0.;
+    for (int i = 0; i < n; i++) {
+        double v2 = s.vx[i] * s.vx[i] + s.vy[i] * s.vy[i] + s.vz[i] * s.vz[i];
+        e += s.m[i] * (0.5 * v2 + s.uene[i] + 0.5 * s.pot[i]);
+    }
+    return e;
+}
+
+void orc_momentum(int n, orc_state s, double out[3])
+{
+    double px = 0., py = 0., pz = 0.;
+    for (int i = 0; i < n; i++) {
+        px += s.m[i] * s.vx[i];
+        py += s.m[i] * s.vy[i];
+        pz += s.m[i] * s.vz[i];
+    }
+    out[0] = px; out[1] = py; out[2] = pz;
+}
+
+double orc_ksr_max(int n, const double *m, const double *x, const double *y, const double *z,
+                   const double *dens, const int32_t *istar, int istar_sel)
+{
+    double mc = 0., xc = 0., yc = 0., zc = 0.;
+    for (int i = 0; i < n; i++) {
+        if (istar == NULL || istar[i] == istar_sel || istar_sel < 0) {
+            mc += m[i];
+            xc += m[i] * x[i];
+            yc += m[i] * y[i];
+            zc += m[i] * z[i];
+        }
+    }
+    double minv = 1. / mc;
+    xc *= minv; yc *= minv; zc *= minv;
+    double r2sum = 0., dnsum = 0.;
+    for (int i = 0; i < n; i++) {
+        double dx = x[i] - xc, dy = y[i] - yc, dz = z[i] - zc;
+        r2sum += dens[i] * (dx * dx + dy * dy + dz * dz);
+        dnsum += dens[i];
+    }
+    return sqrt(r2sum / dnsum);
+}

@@ -1,0 +1,86 @@
+"""Shared inputs for the parity tests: seeded particles, lists, and the
+oracle's per-functor inputs/outputs on them.
+
+Every functor is checked in isolation: the GPU gets exactly the inputs the
+oracle got (the oracle's own upstream outputs), so a mismatch names one
+functor.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+from research_codes_amd import ic
+from research_codes_amd.tree import HostTree, sort_particles
+
+KSRH = 1.936492
+
+
+def polytrope_case(N=2000, seed=1000, eta=1.6, n_group=64, theta=0.5, eps=None, vel=0.3,
+                   hmax=None):
+    """n=1.5 polytrope with random velocities, Morton-sorted, with all three lists.
+
+    eps=None uses the reference's softening eps = H/kappa (hdr_gravity.hpp:136);
+    a float gives a constant softening.
+    """
+    p = ic.polytrope(1.5, N, 1.0, 1.0, seed=seed, eta=eta)
+    rng = np.random.default_rng(seed)
+    for c in ("vx", "vy", "vz"):
+        p[c] = vel * rng.standard_normal(N)
+    p["vx"] += 0.5 * p["y"]          # add shear and rotation so curl/div are non-trivial
+    p["vy"] -= 0.3 * p["x"]
+    p, _ = sort_particles(p)
+    p["rs"] = 1.1 * p["ksr"]
+    if hmax is None:
+        hmax = 10.0
+    case = dict(p=p, eta=eta, hmax=hmax, n_group=n_group, theta=theta, eps=eps)
+
+    # density: gather lists with rs (hdr_density.hpp:376-379)
+    eps2_0 = (p["ksr"] / KSRH) ** 2 if eps is None else np.full(N, eps * eps)
+    tree = HostTree(p["x"], p["y"], p["z"], p["m"], eps2_0, 8, n_group)
+    case["i_off"] = tree.i_off
+    case["dens_lists"] = tree.gather(p["rs"])
+    d = oracle.density(tree.i_off, *case["dens_lists"], p, p, eta=eta, ksrh=KSRH, hmax=hmax)
+    case["dens_out"] = d
+
+    # state after copyFromForce (hdr_density.hpp:24-31), EOS, Balsara
+    s = oracle.make_state(N, m=p["m"], x=p["x"], y=p["y"], z=p["z"], vx=p["vx"], vy=p["vy"],
+                          vz=p["vz"], uene=p["uene"], alph=p["alph"], alphu=p["alphu"])
+    for k in ("dens", "divv", "rotv", "grdh", "ksr"):
+        s[k] = d[k].copy()
+    prm = oracle.params()
+    oracle.eos_balsara(s, prm)
+
+    # gravity with the updated softening
+    eps2 = (s["ksr"] / KSRH) ** 2 if eps is None else np.full(N, eps * eps)
+    gtree = HostTree(p["x"], p["y"], p["z"], p["m"], eps2, 8, n_group)
+    jl = gtree.long(theta)
+    mom = gtree.moments()
+    gsrc = dict(m=p["m"], x=p["x"], y=p["y"], z=p["z"], eps2=eps2)
+    g = oracle.gravity(gtree.i_off, jl[0], jl[1], gsrc, gsrc, s_off=jl[2], s_idx=jl[3], ps=mom)
+    case.update(grav_i_off=gtree.i_off, grav_lists=jl, grav_src=gsrc, grav_mom=mom, grav_out=g)
+    oracle.gravity_writeback(s, prm, g)
+
+    # hydro: symmetry lists with the new H
+    htree = HostTree(p["x"], p["y"], p["z"], p["m"], None, 8, n_group)
+    hl = htree.symmetry(s["ksr"])
+    prep = oracle.hydro_prep(s)
+    hsrc = dict(id=p["id"], m=p["m"], x=p["x"], y=p["y"], z=p["z"], vx=p["vx"], vy=p["vy"],
+                vz=p["vz"], hinv=prep["hinv"], pres=s["pres"], presc=prep["presc"],
+                bswt=prep["bswt_half"], dens=s["dens"], vsnd=s["vsnd"], alph=s["alph"],
+                alphu=s["alphu"], thrm=prep["thrm"], eta=s["eta"])
+    h = oracle.hydro(htree.i_off, *hl, hsrc, hsrc)
+    case.update(hydro_i_off=htree.i_off, hydro_lists=hl, hydro_src=hsrc, hydro_out=h, state=s)
+    for t in (tree, gtree, htree):
+        t.close()
+    return case
+
+
+def scaled_err(got, ref, floor_frac=1e-3):
+    """max |got-ref| / max(|ref|, floor_frac * rms(ref)) -- a relative error
+    that does not blow up where a component crosses zero."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    rms = float(np.sqrt(np.mean(ref * ref))) if ref.size else 0.0
+    den = np.maximum(np.abs(ref), floor_frac * rms + 1e-300)
+    return float(np.max(np.abs(got - ref) / den)) if ref.size else 0.0
