@@ -1,0 +1,251 @@
+"""Benchmark: one SPH force step (density + hydro + gravity) at N = 10^6.
+
+Workload: off-axis parabolic collision of an n=3 polytrope (M=1, R=1) and an
+n=1.5 polytrope (M=0.5, R=0.6), N = 10^6 equal-mass particles in total,
+impact parameter 0.25 R at separation 3 R, ~50 neighbours (eta = 1.18),
+constant softening 0.01, opening angle 0.5, synthetic seeded initial data.
+
+A "step" is one force evaluation with the walk lists resident in HBM: density
+(three FP64 h-iteration passes + grad-h/div/curl), EOS + Balsara, tree gravity
+(particle-particle + particle-monopole), hydro (pressure, AV, conduction,
+du/dt) and the per-particle write-backs.  Lists are built on the host once
+before timing (build time reported separately).
+
+Output: one JSON line.  `value` = interaction-list entries evaluated per
+second by the three interaction kernels in the mixed-precision step;
+`fp64` holds the same step with every pair term in FP64.
+
+Usage: python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SPH step time & interactions/s at N=1e6, mixed vs FP64 GPU; % FP32 peak"
+FLOP_PER_ENTRY = {"density": 120, "hydro": 100, "gravity": 29}
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def list_entries(w) -> int:
+    return int(w.entries + w.entries_spj)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--n-group", type=int, default=16)
+    ap.add_argument("--eta", type=float, default=1.18)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="per functor")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU spatial decomposition "
+              "is a design note, not code); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    from research_codes_amd import ic
+    from research_codes_amd.device import FP64, MIXED, Context
+    from research_codes_amd.step import SPHSystem
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+
+    t0 = time.perf_counter()
+    p = ic.off_axis(args.n, eta=args.eta)
+    log(f"ic N={args.n} in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(0)
+    ctx.timing(True)
+    sysm = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=args.n_group,
+                     precision=MIXED)
+    t0 = time.perf_counter()
+    rounds = sysm.relax_h(3)
+    sysm.eos_balsara()
+    log(f"density converged in {rounds} round(s), {time.perf_counter() - t0:.1f}s")
+    sysm.rs.copy_(1.1 * sysm.s["ksr"])
+    sysm.build_lists(("dens", "hydro", "grav"))
+    list_s = sysm.list_build_s
+    log(f"lists built in {list_s:.2f}s")
+    W = sysm.lists
+    entries = {"density": list_entries(W["dens"]), "hydro": list_entries(W["hydro"]),
+               "gravity": list_entries(W["grav"])}
+    total_entries = sum(entries.values())
+    flops = sum(entries[k] * FLOP_PER_ENTRY[k] for k in entries)
+
+    snap = {k: v.clone() for k, v in sysm.s.items()}
+
+    def restore():
+        for k, v in snap.items():
+            sysm.s[k].copy_(v)
+
+    def run(prec, warm, steps):
+        for _ in range(warm):
+            restore()
+            sysm.force(prec)
+        torch.cuda.synchronize()
+        ctx.reset_kernel_time()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        wall = 0.0
+        for _ in range(steps):
+            restore()
+            torch.cuda.synchronize()
+            ev0.record()
+            t = time.perf_counter()
+            sysm.force(prec)
+            ev1.record()
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t
+        # all kernels of the step run on torch's current stream, so the
+        # torch events bracket the whole step
+        step_ms = ev0.elapsed_time(ev1)  # last step
+        kern = {}
+        for k, which in (("density", 0), ("hydro", 1), ("gravity", 2)):
+            ms, cnt = ctx.kernel_time(which)
+            kern[k] = ms / max(cnt, 1)
+        return wall / steps * 1e3, step_ms, kern
+
+    log("timing mixed")
+    ms_mixed, last_mixed, kern_mixed = run(MIXED, args.warmup, args.steps)
+    acc_mixed = [sysm.s[c].clone() for c in ("ax", "ay", "az")]
+    udot_mixed = sysm.s["udot"].clone()
+    res = {}
+    if not args.no_fp64:
+        log("timing fp64")
+        ms_fp64, last_fp64, kern_fp64 = run(FP64, 1, max(3, args.steps // 4))
+        acc64 = [sysm.s[c] for c in ("ax", "ay", "az")]
+        d = torch.sqrt(sum((a - b) ** 2 for a, b in zip(acc_mixed, acc64)))
+        n64 = torch.sqrt(sum(b ** 2 for b in acc64))
+        rel = (d / n64.clamp_min(1e-300))
+        res["fp64"] = {"ms_per_step": round(ms_fp64, 4), "kernel_ms": {k: round(v, 4) for k, v in kern_fp64.items()},
+                       "value": total_entries / (ms_fp64 * 1e-3)}
+        res["speedup_mixed_vs_fp64"] = round(ms_fp64 / ms_mixed, 3)
+        res["acc_rel_err_mixed_vs_fp64"] = {"max": float(rel.max()), "median": float(rel.median()),
+                                            "rms": float(torch.sqrt((rel ** 2).mean()))}
+        du = (udot_mixed - sysm.s["udot"]).abs()
+        res["udot_err_mixed_vs_fp64_over_rms"] = float(du.max() / torch.sqrt((sysm.s["udot"] ** 2).mean()))
+
+    dom = max(kern_mixed, key=lambda k: kern_mixed[k])
+    achieved = entries[dom] * FLOP_PER_ENTRY[dom] / (kern_mixed[dom] * 1e-3)
+    roofline = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                "flop_per_entry": FLOP_PER_ENTRY[dom], "entries_per_launch": entries[dom]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sysm, entries, args.cpu_seconds, log)
+
+    out = {
+        "metric": METRIC, "value": total_entries / (ms_mixed * 1e-3), "unit": "interactions/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_mixed, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded Lane-Emden polytropes)",
+        "config": {"workload": "off-axis parabolic collision n=3 (M=1,R=1) + n=1.5 (M=0.5,R=0.6), "
+                               "b=0.25R at 3R, one density+hydro+gravity force step",
+                   "N": sysm.n, "eta": args.eta, "neighbours_target": 50, "theta": 0.5,
+                   "softening": 0.01, "n_group": args.n_group, "n_leaf": 8,
+                   "walks": W["dens"].n_walk},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "entries": entries, "flop_per_step": flops,
+        "kernel_ms": {k: round(v, 4) for k, v in kern_mixed.items()},
+        "tflops_step": flops / (ms_mixed * 1e-3) / 1e12,
+        "list_build_s": round(list_s, 3), "density_rounds": rounds,
+        "itr_after_step": int(sysm.itr.sum().item()),
+    }
+    out.update(res)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+def cpu_baseline(sysm, entries, seconds, log):
+    """The oracle (scalar FP64 C, OpenMP over walks) on a bounded sample of the
+    same walks and lists; returns interactions/s scaled to the full step."""
+    import oracle
+    nthreads = min(16, os.cpu_count() or 1)
+    s = {k: v.cpu().numpy() for k, v in sysm.s.items()}
+    ids = sysm.id.cpu().numpy()
+    rs = sysm.rs.cpu().numpy()
+    hp = {k: v.cpu().numpy() for k, v in sysm.hprep.items()}
+    eps2 = sysm.eps2.cpu().numpy()
+    mom = {k: v.cpu().numpy() for k, v in sysm.mom.items()}
+    host = {}
+    for k, w in sysm.lists.items():
+        host[k] = [None if t is None else t.cpu().numpy() for t in (w.i_off, w.j_off, w.j_idx, w.s_off, w.s_idx)]
+    dsrc = dict(id=ids, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"], vz=s["vz"],
+                ksr=s["ksr"], rs=rs)
+    hsrc = dict(id=ids, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"], vz=s["vz"],
+                hinv=hp["hinv"], pres=s["pres"], presc=hp["presc"], bswt=hp["bswt_half"],
+                dens=s["dens"], vsnd=s["vsnd"], alph=s["alph"], alphu=s["alphu"], thrm=hp["thrm"],
+                eta=s["eta"])
+    gsrc = dict(m=s["m"], x=s["x"], y=s["y"], z=s["z"], eps2=eps2)
+
+    def sample(kind, fn):
+        i_off, j_off, j_idx, s_off, s_idx = host[kind]
+        nw = len(i_off) - 1
+        ni = np.diff(i_off.astype(np.int64))
+        per = ni * np.diff(j_off.astype(np.int64))
+        if s_off is not None:
+            per = per + ni * np.diff(s_off.astype(np.int64))
+        done_e, done_t, frac = 0, 0.0, 0.002
+        # 8 evenly spaced blocks of consecutive walks, grown until the budget is used
+        while done_t < seconds and frac <= 1.0:
+            bl = max(1, int(nw * frac / 8))
+            for b in range(8):
+                w0 = min(nw - bl, b * nw // 8)
+                sl = slice(w0, w0 + bl + 1)
+                t = time.perf_counter()
+                fn(i_off[sl], j_off[sl], j_idx, None if s_off is None else s_off[sl], s_idx)
+                done_t += time.perf_counter() - t
+                done_e += int(per[w0:w0 + bl].sum())
+            frac *= 4
+        return done_e, done_t
+
+    res = {}
+    e, t = sample("dens", lambda io, jo, ji, so, si: oracle.density(io, jo, ji, dsrc, dsrc, eta=sysm.eta,
+                  hmax=sysm.ctx.params["hmax"], nthreads=nthreads))
+    res["density"] = (e, t)
+    e, t = sample("hydro", lambda io, jo, ji, so, si: oracle.hydro(io, jo, ji, hsrc, hsrc, nthreads=nthreads))
+    res["hydro"] = (e, t)
+    e, t = sample("grav", lambda io, jo, ji, so, si: oracle.gravity(io, jo, ji, gsrc, gsrc, s_off=so, s_idx=si,
+                  ps=mom, nthreads=nthreads))
+    res["gravity"] = (e, t)
+    step_s = sum(entries[k] / (res[k][0] / res[k][1]) for k in res)
+    total = sum(entries.values())
+    log("cpu baseline: " + ", ".join(f"{k} {res[k][0]:.3g} entries in {res[k][1]:.1f}s" for k in res))
+    return {"value": total / step_s, "unit": "interactions/s", "cores": nthreads, "kind": "port",
+            "cpu": cpu_model(), "ms_per_step_est": round(step_s * 1e3, 1),
+            "sample": "oracle (scalar FP64 C, OpenMP over walks) on 8 evenly spaced blocks of "
+                      "consecutive walks per functor, same lists and inputs as the GPU step; "
+                      + ", ".join(f"{k}: {res[k][0]:.3g} entries / {res[k][1]:.1f}s" for k in res)}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

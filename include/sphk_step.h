@@ -1,0 +1,94 @@
+/*
+ * sphk_step.h -- per-particle step kernels around the interaction functors
+ * (C-ABI of libsphk.so; device pointers, asynchronous on `stream`).
+ *
+ * These are the reference's O(N) host loops between and after the three
+ * force passes of calcSPHKernel (sph_yamauchi/run/main.cpp:293-358) and its
+ * integrator, moved to the device so that a whole step stays in HBM:
+ *   referEquationOfState        hdr_hgas.hpp:318-344 (ideal gas P = rho u / n)
+ *   calcBalsaraSwitch           hdr_sph.hpp:79-89
+ *   GravityEPI/EPJ eps2         hdr_gravity.hpp:134-137,162-166
+ *   SPH::copyFromForce(Gravity) hdr_gravity.hpp:107-113
+ *   HydroEPI::copyFromFP        hdr_hydro.hpp:45-62
+ *   SPH::copyFromForce(Hydro)   hdr_hydro.hpp:19-26
+ *   sumAcceleration             hdr_sph.hpp:95-97
+ *   calcAlphaDot                hdr_hgas.hpp:750-762
+ *   calcTimestep (per particle) hdr_hgas.hpp:477-488
+ *   predict / correct           hdr_hgas.hpp:646-665
+ *   calcEnergy / calcMomentum   hdr_hgas.hpp:630-644, main.cpp:197-239
+ * The ideal-gas minimum energy umin is 0 (hdr_hgas.hpp:342).
+ */
+#ifndef SPHK_STEP_H
+#define SPHK_STEP_H
+
+#include "sphk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Particle state, structure of arrays (FP64, device pointers).  Field
+ * meanings follow class SPH (hdr_sph.hpp:12-50); ax..az is acc, vx2.. the
+ * half-step velocity, g1* accg1 (hydro grad-h gravity), g2* accg2 (tree
+ * gravity). */
+typedef struct {
+    double *m, *x, *y, *z, *vx, *vy, *vz;
+    double *ax, *ay, *az;
+    double *vx2, *vy2, *vz2;
+    double *uene, *uene2, *udot;
+    double *alph, *alph2, *adot;
+    double *alphu, *alphu2, *adotu;
+    double *diffu, *ksr, *dens, *pres, *vsnd, *divv, *rotv, *bswt, *grdh, *vsmx;
+    double *pot, *eta;
+    double *achx, *achy, *achz;
+    double *g1x, *g1y, *g1z;
+    double *g2x, *g2y, *g2z;
+} sphk_state;
+
+/* Physical parameters of the step (RP:: globals and EOS constants). */
+typedef struct {
+    double ninv;        /* 1/n of P = ninv rho u (n = 1.5 <=> gamma = 5/3) */
+    double ksrh;        /* kappa */
+    double G;           /* gravitational constant */
+    double alpha_max, alpha_min, alphu_max, alphu_min;
+    double eps_u;       /* RP::EpsilonOfInternalEnergy */
+    double c_dt;        /* RP::CoefficientOfTimestep */
+    double dt_max;      /* RP::MaximumTimestep */
+} sphk_phys;
+
+/* EOS and Balsara switch.  The density functor's outputs are expected in
+ * the state already (SPH::copyFromForce, hdr_density.hpp:24-31): the
+ * device-resident step points sphk_dens_out at the state arrays. */
+int sphk_eos_balsara_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, void *stream);
+
+/* eps2 = (ksr / kappa)^2 when eps_const <= 0, else eps_const^2. */
+int sphk_softening_dev(sphk_ctx *ctx, int n, const double *ksr, double ksrh, double eps_const,
+                       double *eps2, void *stream);
+
+/* accg2, pot and eta from the gravity functor's outputs. */
+int sphk_gravity_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p,
+                               sphk_grav_out g, void *stream);
+
+/* hinv, presc, 0.5*bswt, thrm for the hydro EPI/EPJ. */
+int sphk_hydro_prep_dev(sphk_ctx *ctx, int n, sphk_state s, double *hinv, double *presc,
+                        double *bswt_half, double *thrm, void *stream);
+
+/* Hydro write-back, acc = acch + accg1 + accg2, alpha-dot. */
+int sphk_hydro_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p,
+                             sphk_hydro_out h, void *stream);
+
+int sphk_predict_dev(sphk_ctx *ctx, int n, sphk_state s, double dt, void *stream);
+int sphk_correct_dev(sphk_ctx *ctx, int n, sphk_state s, double dt, void *stream);
+
+/* FP64 reductions with a fixed (deterministic) order, written to host
+ * memory out[0..5] after a stream synchronisation:
+ *   out[0] total energy  sum m (v^2/2 + u + pot/2)
+ *   out[1..3] momentum   sum m v
+ *   out[4] min over particles of the per-particle timestep
+ *   out[5] total mass */
+int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -48,6 +48,9 @@ int sphk_morton_order(int n, const double *x, const double *y, const double *z, 
 int sphk_tree_build(int n, const double *x, const double *y, const double *z, const double *m,
                     const double *eps2, int n_leaf, int n_group, sphk_tree **out);
 void sphk_tree_free(sphk_tree *t);
+/* Replace the walks by fixed chunks of `size` consecutive (Morton-ordered)
+ * particles, the last one possibly shorter.  Invalidates built lists. */
+int sphk_tree_group_fixed(sphk_tree *t, int size);
 
 int sphk_tree_n_walk(const sphk_tree *t);
 int sphk_tree_n_node(const sphk_tree *t);

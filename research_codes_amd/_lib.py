@@ -51,6 +51,18 @@ HydroOut = _struct("HydroOut", HYDRO_OUT)
 GravSrc = _struct("GravSrc", GRAV_SRC)
 GravOut = _struct("GravOut", GRAV_OUT)
 
+STATE_FIELDS = ("m", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "vx2", "vy2", "vz2",
+                "uene", "uene2", "udot", "alph", "alph2", "adot", "alphu", "alphu2", "adotu",
+                "diffu", "ksr", "dens", "pres", "vsnd", "divv", "rotv", "bswt", "grdh", "vsmx",
+                "pot", "eta", "achx", "achy", "achz", "g1x", "g1y", "g1z", "g2x", "g2y", "g2z")
+State = _struct("State", STATE_FIELDS)
+
+
+class Phys(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("ninv", "ksrh", "G", "alpha_max", "alpha_min",
+                                          "alphu_max", "alphu_min", "eps_u", "c_dt", "dt_max")]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     # sphk.h
@@ -76,10 +88,20 @@ SIGNATURES = {
     "sphk_timing": (C.c_int, [_P, C.c_int]),
     "sphk_kernel_time": (C.c_int, [_P, C.c_int, _DP, C.POINTER(C.c_int)]),
     "sphk_reset_kernel_time": (C.c_int, [_P]),
+    # sphk_step.h
+    "sphk_eos_balsara_dev": (C.c_int, [_P, C.c_int, State, Phys, _P]),
+    "sphk_softening_dev": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P, _P]),
+    "sphk_gravity_writeback_dev": (C.c_int, [_P, C.c_int, State, Phys, GravOut, _P]),
+    "sphk_hydro_prep_dev": (C.c_int, [_P, C.c_int, State, _P, _P, _P, _P, _P]),
+    "sphk_hydro_writeback_dev": (C.c_int, [_P, C.c_int, State, Phys, HydroOut, _P]),
+    "sphk_predict_dev": (C.c_int, [_P, C.c_int, State, C.c_double, _P]),
+    "sphk_correct_dev": (C.c_int, [_P, C.c_int, State, C.c_double, _P]),
+    "sphk_diagnostics": (C.c_int, [_P, C.c_int, State, Phys, _DP, _P]),
     # sphk_tree.h
     "sphk_morton_order": (C.c_int, [C.c_int, _P, _P, _P, _P]),
     "sphk_tree_build": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(_P)]),
     "sphk_tree_free": (None, [_P]),
+    "sphk_tree_group_fixed": (C.c_int, [_P, C.c_int]),
     "sphk_tree_n_walk": (C.c_int, [_P]),
     "sphk_tree_n_node": (C.c_int, [_P]),
     "sphk_tree_i_off": (_I32P, [_P]),

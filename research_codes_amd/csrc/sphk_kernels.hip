@@ -1,19 +1,21 @@
 // sphk_kernels.hip -- gfx950 kernels for the SPH interaction functors.
 //
 // Launch shape (all three kernels): one 64-lane workgroup (one wavefront)
-// per walk.  A walk's i-particles are processed in chunks of <= 64; inside a
-// chunk of ni particles, P = pow2ceil(ni) lanes hold distinct i's and the
-// 64/P lane groups split the walk's j-list between them (entry k of a tile
-// goes to slice k mod S).  j-entries are staged through LDS in tiles of 64:
-// lane k loads entry k (SoA gathers through j_idx, coalesced when the list is
-// a run of consecutive particles), then every lane sweeps its slice of the
-// tile.  Per-i partial sums are combined across slices with a butterfly of
-// cross-lane shuffles.
+// per walk.  A walk's i-particles are processed in chunks of <= 64.  Inside a
+// chunk of ni particles the wave is split into S = floor(64/ni) j-slices:
+// lane l holds i = l mod ni and slice l / ni (lanes beyond S*ni idle), so a
+// walk of 5 particles keeps 60 of 64 lanes busy.  j-entries are staged
+// through LDS in tiles of 64 records: lane k gathers entry k through j_idx
+// (coalesced where the list is a run of consecutive particles) and the tile
+// is padded with inert records, so the sweep k = slice, slice+S, ... < 64
+// has no bounds checks.  Records are packed as double4 / R4 so a lane reads
+// an entry with 128-bit LDS loads.  Per-i partials are combined across
+// slices through LDS in a fixed order (bitwise reproducible).
 //
-// Precision.  Positions are FP64 everywhere; each pair displacement is
-// formed in FP64.  MIXED=false evaluates the pair term in FP64; MIXED=true
-// casts the displacement to FP32, evaluates the pair term in FP32, sums a
-// tile's partials in FP32 and folds them into FP64 per-i totals.  The three
+// Precision.  Positions are FP64; every pair displacement is formed in FP64.
+// MIXED=false evaluates the pair term in FP64; MIXED=true casts the
+// displacement to FP32, evaluates the pair term in FP32, sums a tile's
+// partials in FP32 and folds them into FP64 per-i totals.  The three
 // smoothing-length passes of the density functor are FP64 in both modes.
 
 #include "sphk_device.hpp"
@@ -21,17 +23,115 @@
 
 namespace sphk {
 
+template <bool MIXED>
+using real_t = typename std::conditional<MIXED, float, double>::type;
+
+template <typename R>
+struct alignas(4 * sizeof(R)) R4 {
+    R a, b, c, d;
+};
+
+// Inert padding record: far away (q >> 1 for any realistic H), zero mass,
+// id -1, very negative sound speed (cannot raise vsmx).
+constexpr double kFar = 1e10;
+
+// Lane map of one i-chunk: S = floor(64/ni) slices.
+struct Map {
+    int ni, S, il, slice;
+    bool lead;  // slice 0 of a real i: owns the reduced result
+    __device__ __forceinline__ Map(int ni_, int lane)
+    {
+        ni = ni_;
+        S = kWave / ni;
+        il = lane % ni;
+        slice = lane / ni;
+        if (slice >= S) slice = kTile;  // idle lane: its sweep is empty
+        lead = lane < ni;
+    }
+};
+
+// Sum (or max) of per-lane values over the slices of each i, through LDS.
+// On return the lead lane of each i holds the result.  red has 64*NQ slots.
+template <int NQ>
+__device__ __forceinline__ void slice_reduce(double (&v)[NQ], double *red, const Map &m, int lane,
+                                             int max_mask = 0)
+{
+    __syncthreads();
+    for (int q = 0; q < NQ; q++) red[q * kWave + lane] = v[q];
+    __syncthreads();
+    if (m.lead) {
+        for (int q = 0; q < NQ; q++) {
+            double a = red[q * kWave + m.il];
+            const bool is_max = (max_mask >> q) & 1;
+            for (int t = 1; t < m.S; t++) {
+                const double b = red[q * kWave + m.il + m.ni * t];
+                a = is_max ? (a > b ? a : b) : a + b;
+            }
+            v[q] = a;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Density: scalar calcDensity, sph_yamauchi/run/hdr_density.hpp:254-363.
 // ---------------------------------------------------------------------------
 
-// q = sqrt(r2) * hinv evaluated exactly as the reference's FP64 code does
-// (separate IEEE roundings, no contraction) -- used to decide q < 1 for pairs
-// that lie within a few ulps of the kernel edge.
+// q = sqrt(r2) * hinv with the reference's FP64 rounding sequence (separate
+// IEEE operations, correctly rounded sqrt): decides q < 1 for pairs within a
+// few ulps of the kernel edge.
 __device__ __attribute__((noinline)) double exact_q(double dx, double dy, double dz, double hinv)
 {
     const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
     return __dmul_rn(__builtin_sqrt(r2), hinv);
+}
+
+struct Pass1 {
+    double rh;
+    int nj;
+};
+
+__device__ __forceinline__ void dens_pass1_pair(const double4 &j, double px, double py, double pz,
+                                                double hi, double hi3, Pass1 &a)
+{
+    const double dx = px - j.x;
+    const double dy = py - j.y;
+    const double dz = pz - j.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double q = r2 * rsqrt_f64(r2 > 1e-300 ? r2 : 1e-300) * hi;
+    if (__builtin_fabs(q - 1.) < 1e-13) q = exact_q(dx, dy, dz, hi);
+    a.rh += j.w * hi3 * kernel0th(q);
+    a.nj += (q < 1.) ? 1 : 0;
+}
+
+template <typename R>
+struct Pass2 {
+    R g, dv, rx, ry, rz;
+};
+
+template <typename R>
+__device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v, int32_t jid,
+                                                double px, double py, double pz, R vx, R vy, R vz,
+                                                int32_t id_i, R hi, R hi4, Pass2<R> &a)
+{
+    const R dpx = (R)(px - j.x);
+    const R dpy = (R)(py - j.y);
+    const R dpz = (R)(pz - j.z);
+    const R dvx = vx - v.a;
+    const R dvy = vy - v.b;
+    const R dvz = vz - v.c;
+    const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+    const R ri = (id_i != jid) ? rsqrt_r(r2) : R(0);
+    const R q = r2 * ri * hi;
+    const R kw0 = kernel0th(q);
+    const R kw1 = kernel1st(q);
+    const R mh = v.d * hi4;
+    a.g -= (R(3) * kw0 + q * kw1) * mh;
+    const R dw = mh * kw1 * ri;
+    const R dwx = dw * dpx, dwy = dw * dpy, dwz = dw * dpz;
+    a.dv -= dvx * dwx + dvy * dwy + dvz * dwz;
+    a.rx += dvy * dwz - dvz * dwy;
+    a.ry += dvz * dwx - dvx * dwz;
+    a.rz += dvx * dwy - dvy * dwx;
 }
 
 template <bool MIXED>
@@ -40,10 +140,12 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     const int32_t *__restrict__ j_idx, sphk_dens_src pi, sphk_dens_src pj, sphk_dens_out out,
     double eta_ksrh, double hmax)
 {
-    using R = typename std::conditional<MIXED, float, double>::type;
-    __shared__ double sx[kTile], sy[kTile], sz[kTile], sm[kTile];
-    __shared__ R svx[kTile], svy[kTile], svz[kTile], smr[kTile];
+    using R = real_t<MIXED>;
+    __shared__ double4 sp[kTile];     // x, y, z, m
+    __shared__ R4<R> sv[kTile];       // vx, vy, vz, m
     __shared__ int32_t sid[kTile];
+    __shared__ double red[5 * kWave];
+    __shared__ double shs[kWave];
 
     const int w = blockIdx.x;
     if (w >= n_walk) return;
@@ -52,13 +154,12 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     const int jb = j_off[w], je = j_off[w + 1];
 
     for (int c0 = ib; c0 < ie; c0 += kWave) {
-        const LaneMap L(min(kWave, ie - c0), lane);
-        const int i = c0 + (L.active ? L.il : 0);
+        const Map M(min(kWave, ie - c0), lane);
+        const int i = c0 + M.il;
         const int32_t id_i = pi.id[i];
         const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
         const double m_i = pi.m[i];
         double hs = pi.ksr[i];
-
         double dens = 0.;
         long long np = 0;
 
@@ -66,102 +167,86 @@ __global__ __launch_bounds__(kWave) void density_kernel(
         for (int rep = 0; rep < 3; rep++) {
             const double hi = 1. / hs;
             const double hi3 = hi * hi * hi;
-            double rh = 0.;
-            int nj = 0;
+            Pass1 a{0., 0};
             for (int t0 = jb; t0 < je; t0 += kTile) {
                 const int nt = min(kTile, je - t0);
                 __syncthreads();
                 if (lane < nt) {
                     const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                    sx[lane] = pj.x[j];
-                    sy[lane] = pj.y[j];
-                    sz[lane] = pj.z[j];
-                    sm[lane] = pj.m[j];
+                    sp[lane] = make_double4(pj.x[j], pj.y[j], pj.z[j], pj.m[j]);
+                } else {
+                    sp[lane] = make_double4(kFar, kFar, kFar, 0.);
                 }
                 __syncthreads();
-                for (int k = L.slice; k < nt; k += L.S) {
-                    const double dx = px - sx[k];
-                    const double dy = py - sy[k];
-                    const double dz = pz - sz[k];
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    const double r1 = r2 > 0. ? r2 * rsqrt_f64(r2) : 0.;
-                    double q = r1 * hi;
-                    if (__builtin_fabs(q - 1.) < 1e-13) q = exact_q(dx, dy, dz, hi);
-                    rh += sm[k] * hi3 * kernel0th(q);
-                    nj += (q < 1.) ? 1 : 0;
+                int k = M.slice;
+                for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+                    const double4 j0 = sp[k], j1 = sp[k + M.S], j2 = sp[k + 2 * M.S], j3 = sp[k + 3 * M.S];
+                    dens_pass1_pair(j0, px, py, pz, hi, hi3, a);
+                    dens_pass1_pair(j1, px, py, pz, hi, hi3, a);
+                    dens_pass1_pair(j2, px, py, pz, hi, hi3, a);
+                    dens_pass1_pair(j3, px, py, pz, hi, hi3, a);
                 }
+                for (; k < kTile; k += M.S) dens_pass1_pair(sp[k], px, py, pz, hi, hi3, a);
             }
-            rh = slice_sum(rh, L.P);
-            nj = slice_sum(nj, L.P);
-            double hnew = eta_ksrh * pow(m_i / rh, 1. / 3.);
-            hnew = hnew < hmax ? hnew : hmax;
-            dens = rh;
-            np = nj;
-            hs = hnew;
+            double v[2] = {a.rh, (double)a.nj};
+            slice_reduce<2>(v, red, M, lane);
+            if (M.lead) {
+                dens = v[0];
+                np = (long long)v[1];
+                double hnew = eta_ksrh * pow(m_i / dens, 1. / 3.);
+                shs[M.il] = hnew < hmax ? hnew : hmax;
+            }
+            __syncthreads();
+            hs = shs[M.il];
         }
 
         // -- grad-h, divergence, curl with H_3 (hdr_density.hpp:304-349)
-        const double vx = pi.vx[i], vy = pi.vy[i], vz = pi.vz[i];
         const double hi_d = 1. / hs;
         const R hi = (R)hi_d;
         const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
-        const R rvx = (R)vx, rvy = (R)vy, rvz = (R)vz;
-        double grdh = 0., divv = 0., rotx = 0., roty = 0., rotz = 0.;
+        const R vx = (R)pi.vx[i], vy = (R)pi.vy[i], vz = (R)pi.vz[i];
+        double acc[5] = {0., 0., 0., 0., 0.};
         for (int t0 = jb; t0 < je; t0 += kTile) {
             const int nt = min(kTile, je - t0);
             __syncthreads();
             if (lane < nt) {
                 const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                sx[lane] = pj.x[j];
-                sy[lane] = pj.y[j];
-                sz[lane] = pj.z[j];
-                smr[lane] = (R)pj.m[j];
-                svx[lane] = (R)pj.vx[j];
-                svy[lane] = (R)pj.vy[j];
-                svz[lane] = (R)pj.vz[j];
+                const double mj = pj.m[j];
+                sp[lane] = make_double4(pj.x[j], pj.y[j], pj.z[j], mj);
+                sv[lane] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)mj};
                 sid[lane] = pj.id[j];
+            } else {
+                sp[lane] = make_double4(kFar, kFar, kFar, 0.);
+                sv[lane] = R4<R>{R(0), R(0), R(0), R(0)};
+                sid[lane] = -1;
             }
             __syncthreads();
-            R g = 0, dv = 0, rx = 0, ry = 0, rz = 0;
-            for (int k = L.slice; k < nt; k += L.S) {
-                const R dpx = (R)(px - sx[k]);
-                const R dpy = (R)(py - sy[k]);
-                const R dpz = (R)(pz - sz[k]);
-                const R dvx = rvx - svx[k];
-                const R dvy = rvy - svy[k];
-                const R dvz = rvz - svz[k];
-                const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-                const R ri = (id_i != sid[k]) ? rsqrt_r(r2) : R(0);
-                const R r1 = r2 * ri;
-                const R q = r1 * hi;
-                const R kw0 = kernel0th(q);
-                const R kw1 = kernel1st(q);
-                const R m_j = smr[k];
-                g -= (R(3) * kw0 + q * kw1) * hi4 * m_j;
-                const R dw = m_j * hi4 * kw1 * ri;
-                const R dwx = dw * dpx, dwy = dw * dpy, dwz = dw * dpz;
-                dv -= dvx * dwx + dvy * dwy + dvz * dwz;
-                rx += dvy * dwz - dvz * dwy;
-                ry += dvz * dwx - dvx * dwz;
-                rz += dvx * dwy - dvy * dwx;
+            Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+            int k = M.slice;
+            for (; k + M.S < kTile; k += 2 * M.S) {
+                const double4 p0 = sp[k], p1 = sp[k + M.S];
+                const R4<R> v0 = sv[k], v1 = sv[k + M.S];
+                const int32_t d0 = sid[k], d1 = sid[k + M.S];
+                dens_pass2_pair<R>(p0, v0, d0, px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
+                dens_pass2_pair<R>(p1, v1, d1, px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
             }
-            grdh += (double)g;
-            divv += (double)dv;
-            rotx += (double)rx;
-            roty += (double)ry;
-            rotz += (double)rz;
+            for (; k < kTile; k += M.S)
+                dens_pass2_pair<R>(sp[k], sv[k], sid[k], px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
+            acc[0] += (double)a.g;
+            acc[1] += (double)a.dv;
+            acc[2] += (double)a.rx;
+            acc[3] += (double)a.ry;
+            acc[4] += (double)a.rz;
         }
-        grdh = slice_sum(grdh, L.P);
-        divv = slice_sum(divv, L.P);
-        rotx = slice_sum(rotx, L.P);
-        roty = slice_sum(roty, L.P);
-        rotz = slice_sum(rotz, L.P);
+        slice_reduce<5>(acc, red, M, lane);
 
         // hdr_density.hpp:351-362
-        if (L.active && L.slice == 0) {
+        if (M.lead) {
+            const double grdh = acc[0];
+            double divv = acc[1];
+            const double rot2 = acc[2] * acc[2] + acc[3] * acc[3] + acc[4] * acc[4];
             const double deni = 1. / dens;
             const double omgi = 1. / (1. + hs * deni * grdh / 3.);
-            const double rot2 = rotx * rotx + roty * roty + rotz * rotz;
             double rotv = rot2 != 0. ? __builtin_sqrt(rot2) : 0.;
             rotv *= deni * omgi;
             divv *= deni * omgi;
@@ -179,17 +264,87 @@ __global__ __launch_bounds__(kWave) void density_kernel(
 // ---------------------------------------------------------------------------
 // Hydro: scalar calcHydro, sph_yamauchi/run/hdr_hydro.hpp:320-421.
 // ---------------------------------------------------------------------------
+template <typename R>
+struct HydroI {
+    double px, py, pz;
+    R vx, vy, vz, hi, hi4, pp, pc, bs, dn, cs, al, au, eg, et;
+    int32_t id;
+};
+
+template <typename R>
+struct HydroAcc {
+    R ax, ay, az, ud, df, gx, gy, gz, vs;
+};
+
+template <typename R>
+__device__ __forceinline__ void hydro_pair(const double4 &p, const R4<R> &A, const R4<R> &B,
+                                           const R4<R> &Cc, R eta_j, int32_t jid,
+                                           const HydroI<R> &I, HydroAcc<R> &a)
+{
+    // p = {x, y, z, m}; A = {vx, vy, vz, hinv}; B = {pres, presc, bswt, dens};
+    // Cc = {vsnd, alph, alphu, thrm}
+    const R dpx = (R)(I.px - p.x);
+    const R dpy = (R)(I.py - p.y);
+    const R dpz = (R)(I.pz - p.z);
+    const R dvx = I.vx - A.a;
+    const R dvy = I.vy - A.b;
+    const R dvz = I.vz - A.c;
+    const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+    const R ri = (I.id != jid) ? rsqrt_r(r2) : R(0);
+    const R r1 = r2 * ri;
+    const R hj = A.d;
+    const R q_i = r1 * I.hi;
+    const R q_j = r1 * hj;
+    const R hi4_j = hj * (hj * hj * hj);
+    const R dw_i = I.hi4 * kernel1st(q_i);
+    const R dw_j = hi4_j * kernel1st(q_j);
+    const R m_j = (R)p.w;
+    const R ka = (dw_i + dw_j) * m_j;
+
+    const R rv = dpx * dvx + dpy * dvy + dpz * dvz;
+    const R w_ij = rv * ri;
+    const R w0 = w_ij < R(0) ? w_ij : R(0);
+    const R vs = I.cs + Cc.a - R(3) * w0;
+    const R rhi = rcp_r(I.dn + B.d);
+    const R av0 = (I.bs + B.c) * (I.al + Cc.b) * vs * w0;
+    a.vs = a.vs > vs ? a.vs : vs;
+
+    const R kri = ka * ri;
+    const R ta = (I.pc + B.b - R(0.5) * av0 * rhi) * kri;
+    a.ax -= ta * dpx;
+    a.ay -= ta * dpy;
+    a.az -= ta * dpz;
+
+    const R vsu2 = __builtin_fabs(I.pp - B.a) * rhi * R(2);
+    const R vsu = vsu2 != R(0) ? vsu2 * rsqrt_r(vsu2) : R(0);
+    const R du = I.eg - Cc.d;
+    a.ud += ka * (I.pc * w_ij - rhi * (R(0.25) * av0 * w0 - (I.au + Cc.c) * vsu * du));
+    a.df += rhi * du * kri;
+
+    const R dg = m_j * ri * (I.et * dw_i + eta_j * dw_j);
+    a.gx += dg * dpx;
+    a.gy += dg * dpy;
+    a.gz += dg * dpz;
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(kWave) void hydro_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_hydro_src pi, sphk_hydro_src pj, sphk_hydro_out out)
 {
-    using R = typename std::conditional<MIXED, float, double>::type;
-    __shared__ double sx[kTile], sy[kTile], sz[kTile];
-    __shared__ R svx[kTile], svy[kTile], svz[kTile], sm[kTile], shinv[kTile], spres[kTile],
-        spresc[kTile], sbswt[kTile], sdens[kTile], svsnd[kTile], salph[kTile], salphu[kTile],
-        sthrm[kTile], seta[kTile];
-    __shared__ int32_t sid[kTile];
+    using R = real_t<MIXED>;
+    // tile records and the slice-reduction buffer share one LDS block (the
+    // reduction runs after the last tile)
+    constexpr int kTileBytes = kTile * (sizeof(double4) + 3 * sizeof(R4<R>) + sizeof(R) + 4);
+    constexpr int kRedBytes = 9 * kWave * sizeof(double);
+    __shared__ __attribute__((aligned(32))) char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
+    double4 *sp = reinterpret_cast<double4 *>(smem);
+    R4<R> *sa = reinterpret_cast<R4<R> *>(sp + kTile);
+    R4<R> *sb = sa + kTile;
+    R4<R> *sc = sb + kTile;
+    R *se = reinterpret_cast<R *>(sc + kTile);
+    int32_t *sid = reinterpret_cast<int32_t *>(se + kTile);
+    double *red = reinterpret_cast<double *>(smem);
 
     const int w = blockIdx.x;
     if (w >= n_walk) return;
@@ -198,118 +353,86 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
     const int jb = j_off[w], je = j_off[w + 1];
 
     for (int c0 = ib; c0 < ie; c0 += kWave) {
-        const LaneMap L(min(kWave, ie - c0), lane);
-        const int i = c0 + (L.active ? L.il : 0);
-        const int32_t id_i = pi.id[i];
-        const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
-        const R vx = (R)pi.vx[i], vy = (R)pi.vy[i], vz = (R)pi.vz[i];
-        const double hi_d = pi.hinv[i];
-        const R hi = (R)hi_d;
-        const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
-        const R pp_i = (R)pi.pres[i], pc_i = (R)pi.presc[i], bs_i = (R)pi.bswt[i];
-        const R dn_i = (R)pi.dens[i], cs_i = (R)pi.vsnd[i], al_i = (R)pi.alph[i];
-        const R au_i = (R)pi.alphu[i], eg_i = (R)pi.thrm[i], et_i = (R)pi.eta[i];
+        const Map M(min(kWave, ie - c0), lane);
+        const int i = c0 + M.il;
+        HydroI<R> I;
+        I.px = pi.x[i];
+        I.py = pi.y[i];
+        I.pz = pi.z[i];
+        I.vx = (R)pi.vx[i];
+        I.vy = (R)pi.vy[i];
+        I.vz = (R)pi.vz[i];
+        const double hid = pi.hinv[i];
+        I.hi = (R)hid;
+        I.hi4 = (R)(hid * (hid * hid * hid));
+        I.pp = (R)pi.pres[i];
+        I.pc = (R)pi.presc[i];
+        I.bs = (R)pi.bswt[i];
+        I.dn = (R)pi.dens[i];
+        I.cs = (R)pi.vsnd[i];
+        I.al = (R)pi.alph[i];
+        I.au = (R)pi.alphu[i];
+        I.eg = (R)pi.thrm[i];
+        I.et = (R)pi.eta[i];
+        I.id = pi.id[i];
 
-        double ax = 0., ay = 0., az = 0., udot = 0., difu = 0., gx = 0., gy = 0., gz = 0.;
-        R vsmx = 0;
+        double acc[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+        R vsmx = R(0);
         for (int t0 = jb; t0 < je; t0 += kTile) {
             const int nt = min(kTile, je - t0);
             __syncthreads();
             if (lane < nt) {
                 const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                sx[lane] = pj.x[j];
-                sy[lane] = pj.y[j];
-                sz[lane] = pj.z[j];
-                svx[lane] = (R)pj.vx[j];
-                svy[lane] = (R)pj.vy[j];
-                svz[lane] = (R)pj.vz[j];
-                sm[lane] = (R)pj.m[j];
-                shinv[lane] = (R)pj.hinv[j];
-                spres[lane] = (R)pj.pres[j];
-                spresc[lane] = (R)pj.presc[j];
-                sbswt[lane] = (R)pj.bswt[j];
-                sdens[lane] = (R)pj.dens[j];
-                svsnd[lane] = (R)pj.vsnd[j];
-                salph[lane] = (R)pj.alph[j];
-                salphu[lane] = (R)pj.alphu[j];
-                sthrm[lane] = (R)pj.thrm[j];
-                seta[lane] = (R)pj.eta[j];
+                sp[lane] = make_double4(pj.x[j], pj.y[j], pj.z[j], pj.m[j]);
+                sa[lane] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)pj.hinv[j]};
+                sb[lane] = R4<R>{(R)pj.pres[j], (R)pj.presc[j], (R)pj.bswt[j], (R)pj.dens[j]};
+                sc[lane] = R4<R>{(R)pj.vsnd[j], (R)pj.alph[j], (R)pj.alphu[j], (R)pj.thrm[j]};
+                se[lane] = (R)pj.eta[j];
                 sid[lane] = pj.id[j];
+            } else {
+                sp[lane] = make_double4(kFar, kFar, kFar, 0.);
+                sa[lane] = R4<R>{R(0), R(0), R(0), R(0)};
+                sb[lane] = R4<R>{R(0), R(0), R(0), R(1)};
+                sc[lane] = R4<R>{R(-1e20), R(0), R(0), R(0)};
+                se[lane] = R(0);
+                sid[lane] = -1;
             }
             __syncthreads();
-            R tax = 0, tay = 0, taz = 0, tud = 0, tdf = 0, tgx = 0, tgy = 0, tgz = 0;
-            for (int k = L.slice; k < nt; k += L.S) {
-                const R dpx = (R)(px - sx[k]);
-                const R dpy = (R)(py - sy[k]);
-                const R dpz = (R)(pz - sz[k]);
-                const R dvx = vx - svx[k];
-                const R dvy = vy - svy[k];
-                const R dvz = vz - svz[k];
-                const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-                const R ri = (id_i != sid[k]) ? rsqrt_r(r2) : R(0);
-                const R r1 = r2 * ri;
-                const R q_i = r1 * hi;
-                const R hj = shinv[k];
-                const R q_j = r1 * hj;
-                const R hi4_j = hj * (hj * hj * hj);
-                const R dw_i = hi4 * kernel1st(q_i);
-                const R dw_j = hi4_j * kernel1st(q_j);
-                const R m_j = sm[k];
-                const R ka = (dw_i + dw_j) * m_j;
-
-                const R rv = dpx * dvx + dpy * dvy + dpz * dvz;
-                const R w_ij = rv * ri;
-                const R w0 = w_ij < R(0) ? w_ij : R(0);
-                const R vs = cs_i + svsnd[k] - R(3) * w0;
-                const R rhi = rcp_r(dn_i + sdens[k]);
-                const R av0 = (bs_i + sbswt[k]) * (al_i + salph[k]) * vs * w0;
-                vsmx = vsmx > vs ? vsmx : vs;
-
-                const R ta = (pc_i + spresc[k] - R(0.5) * av0 * rhi) * ka * ri;
-                tax -= ta * dpx;
-                tay -= ta * dpy;
-                taz -= ta * dpz;
-
-                const R vsu2 = __builtin_fabs(pp_i - spres[k]) * rhi * R(2);
-                const R vsu = vsu2 != R(0) ? vsu2 * rsqrt_r(vsu2) : R(0);
-                const R du = eg_i - sthrm[k];
-                tud += ka * (pc_i * w_ij - rhi * (R(0.25) * av0 * w0 - (au_i + salphu[k]) * vsu * du));
-                tdf += rhi * du * ka * ri;
-
-                const R dg = m_j * ri * (et_i * dw_i + seta[k] * dw_j);
-                tgx += dg * dpx;
-                tgy += dg * dpy;
-                tgz += dg * dpz;
+            HydroAcc<R> a{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), vsmx};
+            int k = M.slice;
+            for (; k + M.S < kTile; k += 2 * M.S) {
+                const int k1 = k + M.S;
+                const double4 p0 = sp[k], p1 = sp[k1];
+                const R4<R> a0 = sa[k], a1 = sa[k1], b0 = sb[k], b1 = sb[k1], c0_ = sc[k], c1 = sc[k1];
+                const R e0 = se[k], e1 = se[k1];
+                const int32_t d0 = sid[k], d1 = sid[k1];
+                hydro_pair<R>(p0, a0, b0, c0_, e0, d0, I, a);
+                hydro_pair<R>(p1, a1, b1, c1, e1, d1, I, a);
             }
-            ax += (double)tax;
-            ay += (double)tay;
-            az += (double)taz;
-            udot += (double)tud;
-            difu += (double)tdf;
-            gx += (double)tgx;
-            gy += (double)tgy;
-            gz += (double)tgz;
+            for (; k < kTile; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sid[k], I, a);
+            acc[0] += (double)a.ax;
+            acc[1] += (double)a.ay;
+            acc[2] += (double)a.az;
+            acc[3] += (double)a.ud;
+            acc[4] += (double)a.df;
+            acc[5] += (double)a.gx;
+            acc[6] += (double)a.gy;
+            acc[7] += (double)a.gz;
+            vsmx = a.vs;
         }
-        ax = slice_sum(ax, L.P);
-        ay = slice_sum(ay, L.P);
-        az = slice_sum(az, L.P);
-        udot = slice_sum(udot, L.P);
-        difu = slice_sum(difu, L.P);
-        gx = slice_sum(gx, L.P);
-        gy = slice_sum(gy, L.P);
-        gz = slice_sum(gz, L.P);
-        vsmx = slice_max(vsmx, L.P);
+        acc[8] = (double)vsmx;
+        slice_reduce<9>(acc, red, M, lane, 1 << 8);
         // hdr_hydro.hpp:402-419
-        if (L.active && L.slice == 0) {
-            out.ax[i] = ax * 0.5;
-            out.ay[i] = ay * 0.5;
-            out.az[i] = az * 0.5;
-            out.udot[i] = udot * 0.5;
-            out.vsmx[i] = (double)vsmx;
-            out.diffu[i] = difu * 2.;
-            out.gx[i] = gx * 0.5;
-            out.gy[i] = gy * 0.5;
-            out.gz[i] = gz * 0.5;
+        if (M.lead) {
+            out.ax[i] = acc[0] * 0.5;
+            out.ay[i] = acc[1] * 0.5;
+            out.az[i] = acc[2] * 0.5;
+            out.udot[i] = acc[3] * 0.5;
+            out.diffu[i] = acc[4] * 2.;
+            out.gx[i] = acc[5] * 0.5;
+            out.gy[i] = acc[6] * 0.5;
+            out.gz[i] = acc[7] * 0.5;
+            out.vsmx[i] = acc[8];
         }
     }
 }
@@ -320,53 +443,67 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
 // ---------------------------------------------------------------------------
 template <typename R>
 struct GravAcc {
-    double ax = 0., ay = 0., az = 0., pot = 0., eta = 0.;
+    R ax, ay, az, pt, et;
 };
 
-template <bool MIXED>
-__device__ __forceinline__ void gravity_list(
-    int lane, const LaneMap &L, int kb, int ke, const int32_t *__restrict__ idx, sphk_grav_src src,
-    double px, double py, double pz, double e2_i, double *sx, double *sy, double *sz,
-    typename std::conditional<MIXED, float, double>::type *sm,
-    typename std::conditional<MIXED, float, double>::type *se, GravAcc<double> &acc)
+template <typename R>
+__device__ __forceinline__ void gravity_pair(const double4 &p, R e2j, double px, double py,
+                                             double pz, R e2i, GravAcc<R> &a)
 {
-    using R = typename std::conditional<MIXED, float, double>::type;
+    // p = {x, y, z, m}
+    const R dpx = (R)(px - p.x);
+    const R dpy = (R)(py - p.y);
+    const R dpz = (R)(pz - p.z);
+    const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+    const R rei_i = rsqrt_r(r2 + e2i);
+    const R rei_j = rsqrt_r(r2 + e2j);
+    const R rei3_i = rei_i * rei_i * rei_i;
+    const R rei3_j = rei_j * rei_j * rei_j;
+    const R m_j = (R)p.w;
+    const R dg2 = m_j * (rei3_i + rei3_j);
+    a.pt -= m_j * (rei_i + rei_j);
+    a.ax -= dpx * dg2;
+    a.ay -= dpy * dg2;
+    a.az -= dpz * dg2;
+    a.et += m_j * rei3_i;
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int ke,
+                                             const int32_t *__restrict__ idx, sphk_grav_src src,
+                                             double px, double py, double pz, double e2_i,
+                                             double4 *sp, real_t<MIXED> *se, double (&acc)[5])
+{
+    using R = real_t<MIXED>;
     const R e2 = (R)e2_i;
     for (int t0 = kb; t0 < ke; t0 += kTile) {
         const int nt = min(kTile, ke - t0);
         __syncthreads();
         if (lane < nt) {
             const int j = idx ? idx[t0 + lane] : t0 + lane;
-            sx[lane] = src.x[j];
-            sy[lane] = src.y[j];
-            sz[lane] = src.z[j];
-            sm[lane] = (R)src.m[j];
+            sp[lane] = make_double4(src.x[j], src.y[j], src.z[j], src.m[j]);
             se[lane] = (R)src.eps2[j];
+        } else {
+            sp[lane] = make_double4(kFar, kFar, kFar, 0.);
+            se[lane] = R(1);
         }
         __syncthreads();
-        R tax = 0, tay = 0, taz = 0, tpt = 0, tet = 0;
-        for (int k = L.slice; k < nt; k += L.S) {
-            const R dpx = (R)(px - sx[k]);
-            const R dpy = (R)(py - sy[k]);
-            const R dpz = (R)(pz - sz[k]);
-            const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-            const R rei_i = rsqrt_r(r2 + e2);
-            const R rei3_i = rei_i * rei_i * rei_i;
-            const R rei_j = rsqrt_r(r2 + se[k]);
-            const R rei3_j = rei_j * rei_j * rei_j;
-            const R m_j = sm[k];
-            const R dg2 = m_j * (rei3_i + rei3_j);
-            tpt -= m_j * (rei_i + rei_j);
-            tax -= dpx * dg2;
-            tay -= dpy * dg2;
-            taz -= dpz * dg2;
-            tet += m_j * rei3_i;
+        GravAcc<R> a{R(0), R(0), R(0), R(0), R(0)};
+        int k = M.slice;
+        for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+            const double4 p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
+            const R e0 = se[k], e1 = se[k + M.S], e2_ = se[k + 2 * M.S], e3 = se[k + 3 * M.S];
+            gravity_pair<R>(p0, e0, px, py, pz, e2, a);
+            gravity_pair<R>(p1, e1, px, py, pz, e2, a);
+            gravity_pair<R>(p2, e2_, px, py, pz, e2, a);
+            gravity_pair<R>(p3, e3, px, py, pz, e2, a);
         }
-        acc.ax += (double)tax;
-        acc.ay += (double)tay;
-        acc.az += (double)taz;
-        acc.pot += (double)tpt;
-        acc.eta += (double)tet;
+        for (; k < kTile; k += M.S) gravity_pair<R>(sp[k], se[k], px, py, pz, e2, a);
+        acc[0] += (double)a.ax;
+        acc[1] += (double)a.ay;
+        acc[2] += (double)a.az;
+        acc[3] += (double)a.pt;
+        acc[4] += (double)a.et;
     }
 }
 
@@ -376,9 +513,13 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
     const int32_t *__restrict__ j_idx, sphk_grav_src pj, const int32_t *__restrict__ s_off,
     const int32_t *__restrict__ s_idx, sphk_grav_src ps, sphk_grav_src pi, sphk_grav_out out)
 {
-    using R = typename std::conditional<MIXED, float, double>::type;
-    __shared__ double sx[kTile], sy[kTile], sz[kTile];
-    __shared__ R sm[kTile], se[kTile];
+    using R = real_t<MIXED>;
+    constexpr int kTileBytes = kTile * (sizeof(double4) + sizeof(R));
+    constexpr int kRedBytes = 5 * kWave * sizeof(double);
+    __shared__ __attribute__((aligned(32))) char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
+    double4 *sp = reinterpret_cast<double4 *>(smem);
+    R *se = reinterpret_cast<R *>(sp + kTile);
+    double *red = reinterpret_cast<double *>(smem);
 
     const int w = blockIdx.x;
     if (w >= n_walk) return;
@@ -386,26 +527,21 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
     const int ib = i_off[w], ie = i_off[w + 1];
 
     for (int c0 = ib; c0 < ie; c0 += kWave) {
-        const LaneMap L(min(kWave, ie - c0), lane);
-        const int i = c0 + (L.active ? L.il : 0);
+        const Map M(min(kWave, ie - c0), lane);
+        const int i = c0 + M.il;
         const double px = pi.x[i], py = pi.y[i], pz = pi.z[i], e2 = pi.eps2[i];
-        GravAcc<double> acc;
-        gravity_list<MIXED>(lane, L, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sx, sy, sz,
-                            sm, se, acc);
+        double acc[5] = {0., 0., 0., 0., 0.};
+        gravity_list<MIXED>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp, se, acc);
         if (s_off)
-            gravity_list<MIXED>(lane, L, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2, sx, sy,
-                                sz, sm, se, acc);
-        acc.ax = slice_sum(acc.ax, L.P);
-        acc.ay = slice_sum(acc.ay, L.P);
-        acc.az = slice_sum(acc.az, L.P);
-        acc.pot = slice_sum(acc.pot, L.P);
-        acc.eta = slice_sum(acc.eta, L.P);
-        if (L.active && L.slice == 0) {
-            out.ax[i] = acc.ax * 0.5;
-            out.ay[i] = acc.ay * 0.5;
-            out.az[i] = acc.az * 0.5;
-            out.pot[i] = acc.pot * 0.5;
-            out.eta[i] = acc.eta;
+            gravity_list<MIXED>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2, sp, se,
+                                acc);
+        slice_reduce<5>(acc, red, M, lane);
+        if (M.lead) {
+            out.ax[i] = acc[0] * 0.5;
+            out.ay[i] = acc[1] * 0.5;
+            out.az[i] = acc[2] * 0.5;
+            out.pot[i] = acc[3] * 0.5;
+            out.eta[i] = acc[4];
         }
     }
 }

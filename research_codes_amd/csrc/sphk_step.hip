@@ -1,0 +1,368 @@
+// sphk_step.hip -- per-particle kernels of one SPH step (include/sphk_step.h).
+//
+// All are HBM-bound elementwise passes: one thread per particle, 256-thread
+// workgroups, FP64 throughout.  The diagnostics reduction is two-stage with
+// a fixed tree order (per-block partials, then one block), so repeated runs
+// are bitwise identical.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "sphk.h"
+#include "sphk_internal.hpp"
+#include "sphk_step.h"
+
+namespace sphk {
+
+constexpr int kBlock = 256;
+
+static int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+__global__ __launch_bounds__(kBlock) void eos_balsara_kernel(int n, sphk_state s, sphk_phys p)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double dens = s.dens[i];
+    const double pres = p.ninv * dens * s.uene[i];               // hdr_hgas.hpp:339
+    const double vsnd = sqrt((1. + p.ninv) * pres / dens);       // hdr_hgas.hpp:341
+    const double ad = fabs(s.divv[i]);
+    s.pres[i] = pres;
+    s.vsnd[i] = vsnd;
+    s.bswt[i] = ad / (ad + fabs(s.rotv[i]) + 1e-4 * vsnd * p.ksrh / s.ksr[i]);  // hdr_sph.hpp:80-81
+}
+
+__global__ __launch_bounds__(kBlock) void softening_kernel(int n, const double *ksr, double kinv,
+                                                           double e2c, double *eps2)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (e2c > 0.) {
+        eps2[i] = e2c;
+    } else {
+        const double h = ksr[i];
+        eps2[i] = h * h * kinv * kinv;                            // hdr_gravity.hpp:136
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void gravity_writeback_kernel(int n, sphk_state s, sphk_phys p,
+                                                                   sphk_grav_out g)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double kinv = 1. / p.ksrh;
+    const double h = s.ksr[i];
+    s.g2x[i] = p.G * g.ax[i];
+    s.g2y[i] = p.G * g.ay[i];
+    s.g2z[i] = p.G * g.az[i];
+    s.eta[i] = h * h * kinv * kinv * s.grdh[i] / (3. * s.dens[i]) * g.eta[i];  // hdr_gravity.hpp:110-111
+    s.pot[i] = p.G * g.pot[i];
+}
+
+__global__ __launch_bounds__(kBlock) void hydro_prep_kernel(int n, sphk_state s, double *hinv,
+                                                            double *presc, double *bswt_half,
+                                                            double *thrm)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double d = s.dens[i];
+    hinv[i] = 1. / s.ksr[i];                                      // hdr_hydro.hpp:50-60
+    presc[i] = s.pres[i] * s.grdh[i] / (d * d);
+    bswt_half[i] = 0.5 * s.bswt[i];
+    const double u = s.uene[i];
+    thrm[i] = u > 0. ? u : 0.;
+}
+
+__global__ __launch_bounds__(kBlock) void hydro_writeback_kernel(int n, sphk_state s, sphk_phys p,
+                                                                 sphk_hydro_out h)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    // hdr_hydro.hpp:21-25
+    const double ahx = h.ax[i], ahy = h.ay[i], ahz = h.az[i];
+    const double g1x = h.gx[i] * p.G, g1y = h.gy[i] * p.G, g1z = h.gz[i] * p.G;
+    s.achx[i] = ahx;
+    s.achy[i] = ahy;
+    s.achz[i] = ahz;
+    s.g1x[i] = g1x;
+    s.g1y[i] = g1y;
+    s.g1z[i] = g1z;
+    s.udot[i] = h.udot[i];
+    s.vsmx[i] = h.vsmx[i];
+    const double diffu = h.diffu[i];
+    s.diffu[i] = diffu;
+    // hdr_sph.hpp:95-97
+    s.ax[i] = ahx + g1x + s.g2x[i];
+    s.ay[i] = ahy + g1y + s.g2y[i];
+    s.az[i] = ahz + g1z + s.g2z[i];
+    // hdr_hgas.hpp:750-762
+    const double alph = s.alph[i], alphu = s.alphu[i], ksr = s.ksr[i];
+    double src = -s.divv[i] * (p.alpha_max - alph);
+    src = src > 0. ? src : 0.;
+    const double tauinv = (0.25 * p.ksrh * s.vsnd[i]) / ksr;
+    s.adot[i] = -(alph - p.alpha_min) * tauinv + src;
+    double u = s.uene[i];
+    u = u > 0. ? u : 0.;
+    double lim = p.alphu_max - alphu;
+    lim = lim > 0. ? lim : 0.;
+    const double srcu = ksr / p.ksrh * fabs(diffu) / sqrt(u + p.eps_u) * lim;
+    s.adotu[i] = -(alphu - p.alphu_min) * tauinv + srcu;
+}
+
+__global__ __launch_bounds__(kBlock) void predict_kernel(int n, sphk_state s, double dt)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    // hdr_hgas.hpp:646-657
+    const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
+    const double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+    s.x[i] = s.x[i] + vx * dt + 0.5 * ax * dt * dt;
+    s.y[i] = s.y[i] + vy * dt + 0.5 * ay * dt * dt;
+    s.z[i] = s.z[i] + vz * dt + 0.5 * az * dt * dt;
+    s.vx2[i] = vx + 0.5 * ax * dt;
+    s.vy2[i] = vy + 0.5 * ay * dt;
+    s.vz2[i] = vz + 0.5 * az * dt;
+    s.vx[i] = vx + ax * dt;
+    s.vy[i] = vy + ay * dt;
+    s.vz[i] = vz + az * dt;
+    const double u = s.uene[i], ud = s.udot[i];
+    s.uene2[i] = u + 0.5 * ud * dt;
+    s.uene[i] = u + ud * dt;
+    const double a = s.alph[i], ad = s.adot[i];
+    s.alph2[i] = a + 0.5 * ad * dt;
+    s.alph[i] = a + ad * dt;
+    const double au = s.alphu[i], aud = s.adotu[i];
+    s.alphu2[i] = au + 0.5 * aud * dt;
+    s.alphu[i] = au + aud * dt;
+}
+
+__global__ __launch_bounds__(kBlock) void correct_kernel(int n, sphk_state s, double dt)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    // hdr_hgas.hpp:659-665
+    s.vx[i] = s.vx2[i] + 0.5 * s.ax[i] * dt;
+    s.vy[i] = s.vy2[i] + 0.5 * s.ay[i] * dt;
+    s.vz[i] = s.vz2[i] + 0.5 * s.az[i] * dt;
+    s.uene[i] = s.uene2[i] + 0.5 * s.udot[i] * dt;
+    s.alph[i] = s.alph2[i] + 0.5 * s.adot[i] * dt;
+    s.alphu[i] = s.alphu2[i] + 0.5 * s.adotu[i] * dt;
+}
+
+// Per-particle terms of the diagnostics: E, Px, Py, Pz, dt_i, m.
+__device__ __forceinline__ void diag_terms(const sphk_state &s, const sphk_phys &p, int i, double v[6])
+{
+    const double m = s.m[i];
+    const double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+    v[0] = m * (0.5 * (vx * vx + vy * vy + vz * vz) + s.uene[i] + 0.5 * s.pot[i]);
+    v[1] = m * vx;
+    v[2] = m * vy;
+    v[3] = m * vz;
+    // hdr_hgas.hpp:477-488
+    const double ksr = s.ksr[i];
+    const double dth = p.c_dt * ksr / (s.vsmx[i] * p.ksrh);
+    const double divv = s.divv[i];
+    const double dtc = divv < 0. ? -p.c_dt / divv : p.dt_max;
+    const double gx = s.g1x[i] + s.g2x[i], gy = s.g1y[i] + s.g2y[i], gz = s.g1z[i] + s.g2z[i];
+    const double dtg = p.c_dt * sqrt(ksr / sqrt(gx * gx + gy * gy + gz * gz));
+    double a = dth < p.dt_max ? dth : p.dt_max;
+    double b = dtg < dtc ? dtg : dtc;
+    v[4] = a < b ? a : b;
+    v[5] = m;
+}
+
+__device__ __forceinline__ void block_reduce6(double v[6], double *sh)
+{
+    const int t = threadIdx.x;
+    for (int k = 0; k < 6; k++) sh[k * kBlock + t] = v[k];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 6; k++) {
+                const double a = sh[k * kBlock + t], b = sh[k * kBlock + t + w];
+                sh[k * kBlock + t] = (k == 4) ? (a < b ? a : b) : a + b;
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < 6; k++) v[k] = sh[k * kBlock];
+}
+
+__global__ __launch_bounds__(kBlock) void diag_partial_kernel(int n, sphk_state s, sphk_phys p,
+                                                              double *part)
+{
+    __shared__ double sh[6 * kBlock];
+    double acc[6] = {0., 0., 0., 0., 1e300, 0.};
+    // grid-stride with a fixed assignment: block b handles i = b*kBlock + t + k*stride
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        double v[6];
+        diag_terms(s, p, i, v);
+        for (int k = 0; k < 6; k++) acc[k] = (k == 4) ? (acc[k] < v[k] ? acc[k] : v[k]) : acc[k] + v[k];
+    }
+    block_reduce6(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) part[k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+__global__ __launch_bounds__(kBlock) void diag_final_kernel(int nb, const double *part, double *out)
+{
+    __shared__ double sh[6 * kBlock];
+    double acc[6] = {0., 0., 0., 0., 1e300, 0.};
+    for (int b = threadIdx.x; b < nb; b += kBlock)
+        for (int k = 0; k < 6; k++) {
+            const double v = part[k * nb + b];
+            acc[k] = (k == 4) ? (acc[k] < v ? acc[k] : v) : acc[k] + v;
+        }
+    block_reduce6(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) out[k] = acc[k];
+}
+
+}  // namespace sphk
+
+using namespace sphk;
+
+namespace {
+
+hipStream_t stream_of(sphk_ctx *ctx, void *stream)
+{
+    return stream ? reinterpret_cast<hipStream_t>(stream)
+                  : reinterpret_cast<hipStream_t>(sphk_stream(ctx));
+}
+
+int launch_check(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(SPHK_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    return SPHK_OK;
+}
+
+bool state_ok(const sphk_state &s)
+{
+    const void *const *f = reinterpret_cast<const void *const *>(&s);
+    for (size_t k = 0; k < sizeof(sphk_state) / sizeof(void *); k++)
+        if (!f[k]) return false;
+    return true;
+}
+
+int pre(sphk_ctx *ctx, int n, const sphk_state *s)
+{
+    if (!ctx) return set_error(SPHK_E_ARG, "null context");
+    if (n < 0) return set_error(SPHK_E_ARG, "negative particle count");
+    if (s && !state_ok(*s)) return set_error(SPHK_E_ARG, "null state field");
+    return SPHK_OK;
+}
+
+// Scratch for the diagnostics partials: one per process and device ordinal.
+struct DiagScratch {
+    double *part = nullptr;
+    double *out = nullptr;
+    int cap = 0;
+    int device = -1;
+};
+thread_local DiagScratch g_diag;
+
+}  // namespace
+
+extern "C" {
+
+int sphk_eos_balsara_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc || n == 0) return rc;
+    hipLaunchKernelGGL(eos_balsara_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n, s, p);
+    return launch_check("eos_balsara");
+}
+
+int sphk_softening_dev(sphk_ctx *ctx, int n, const double *ksr, double ksrh, double eps_const,
+                       double *eps2, void *stream)
+{
+    int rc = pre(ctx, n, nullptr);
+    if (rc || n == 0) return rc;
+    if (!ksr || !eps2 || !(ksrh > 0.)) return set_error(SPHK_E_ARG, "softening: bad argument");
+    const double e2c = eps_const > 0. ? eps_const * eps_const : 0.;
+    hipLaunchKernelGGL(softening_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n, ksr,
+                       1. / ksrh, e2c, eps2);
+    return launch_check("softening");
+}
+
+int sphk_gravity_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, sphk_grav_out g,
+                               void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc || n == 0) return rc;
+    if (!g.ax || !g.ay || !g.az || !g.pot || !g.eta) return set_error(SPHK_E_ARG, "gravity_writeback: null");
+    hipLaunchKernelGGL(gravity_writeback_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n,
+                       s, p, g);
+    return launch_check("gravity_writeback");
+}
+
+int sphk_hydro_prep_dev(sphk_ctx *ctx, int n, sphk_state s, double *hinv, double *presc,
+                        double *bswt_half, double *thrm, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc || n == 0) return rc;
+    if (!hinv || !presc || !bswt_half || !thrm) return set_error(SPHK_E_ARG, "hydro_prep: null");
+    hipLaunchKernelGGL(hydro_prep_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n, s,
+                       hinv, presc, bswt_half, thrm);
+    return launch_check("hydro_prep");
+}
+
+int sphk_hydro_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, sphk_hydro_out h,
+                             void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc || n == 0) return rc;
+    const void *const *f = reinterpret_cast<const void *const *>(&h);
+    for (int k = 0; k < 9; k++)
+        if (!f[k]) return set_error(SPHK_E_ARG, "hydro_writeback: null field %d", k);
+    hipLaunchKernelGGL(hydro_writeback_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n,
+                       s, p, h);
+    return launch_check("hydro_writeback");
+}
+
+int sphk_predict_dev(sphk_ctx *ctx, int n, sphk_state s, double dt, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc || n == 0) return rc;
+    hipLaunchKernelGGL(predict_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n, s, dt);
+    return launch_check("predict");
+}
+
+int sphk_correct_dev(sphk_ctx *ctx, int n, sphk_state s, double dt, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc || n == 0) return rc;
+    hipLaunchKernelGGL(correct_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n, s, dt);
+    return launch_check("correct");
+}
+
+int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *out, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc) return rc;
+    if (!out) return set_error(SPHK_E_ARG, "diagnostics: null output");
+    const int dev = sphk_device(ctx);
+    const int nb = 1024;
+    if (g_diag.device != dev || g_diag.cap < nb) {
+        if (g_diag.part) (void)hipFree(g_diag.part);
+        if (g_diag.out) (void)hipFree(g_diag.out);
+        g_diag = DiagScratch();
+        if (hipMalloc(&g_diag.part, sizeof(double) * 6 * nb) != hipSuccess ||
+            hipMalloc(&g_diag.out, sizeof(double) * 6) != hipSuccess)
+            return set_error(SPHK_E_DEVICE, "diagnostics: hipMalloc failed");
+        g_diag.cap = nb;
+        g_diag.device = dev;
+    }
+    hipStream_t st = stream_of(ctx, stream);
+    hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(kBlock), 0, st, n, s, p, g_diag.part);
+    if ((rc = launch_check("diagnostics"))) return rc;
+    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(kBlock), 0, st, nb, g_diag.part, g_diag.out);
+    if ((rc = launch_check("diagnostics"))) return rc;
+    if (hipMemcpyAsync(out, g_diag.out, sizeof(double) * 6, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "diagnostics: copy-back failed");
+    return SPHK_OK;
+}
+
+}  // extern "C"

@@ -133,7 +133,8 @@ struct sphk_tree {
     std::vector<uint64_t> key;
     std::vector<Node> node;
     std::vector<double> mm, mx, my, mz, me;  // node monopoles
-    std::vector<int32_t> group;               // node index per walk
+    std::vector<int32_t> group;               // node index per walk (tree groups)
+    std::vector<Box> gbox;                    // tight box of each walk's particles
     std::vector<int32_t> i_off;
     Lists lists[3];
     bool built[3] = {false, false, false};
@@ -277,9 +278,36 @@ int sphk_tree_build(int n, const double *x, const double *y, const double *z, co
         }
     }
     t->i_off.resize(t->group.size() + 1);
-    for (size_t g = 0; g < t->group.size(); g++) t->i_off[g] = t->node[t->group[g]].begin;
+    t->gbox.resize(t->group.size());
+    for (size_t g = 0; g < t->group.size(); g++) {
+        t->i_off[g] = t->node[t->group[g]].begin;
+        t->gbox[g] = t->node[t->group[g]].box;
+    }
     t->i_off[t->group.size()] = n;
     *out = t;
+    return SPHK_OK;
+}
+
+int sphk_tree_group_fixed(sphk_tree *t, int size)
+{
+    if (!t || size < 1) return set_error(SPHK_E_ARG, "group_fixed: bad argument");
+    const int n = t->n;
+    const int ng = (n + size - 1) / size;
+    t->group.assign(ng, -1);
+    t->gbox.resize(ng);
+    t->i_off.resize(ng + 1);
+    for (int g = 0; g < ng; g++) {
+        t->i_off[g] = g * size;
+        Box b;
+        b.empty();
+        for (int i = g * size; i < std::min(n, (g + 1) * size); i++) {
+            const double p[3] = {t->x[i], t->y[i], t->z[i]};
+            b.add(p, 0.);
+        }
+        t->gbox[g] = b;
+    }
+    t->i_off[ng] = n;
+    for (auto &b : t->built) b = false;
     return SPHK_OK;
 }
 
@@ -339,10 +367,9 @@ int sphk_tree_list_gather(sphk_tree *t, const double *rs)
 {
     if (!t || !rs) return set_error(SPHK_E_ARG, "gather: null argument");
     int rc = build_lists(t, t->lists[0], false, [&](int g, std::vector<int32_t> &out, std::vector<int32_t> *) {
-        const Node &gn = t->node[t->group[g]];
         Box ob;
         ob.empty();
-        for (int i = gn.begin; i < gn.end; i++) {
+        for (int i = t->i_off[g]; i < t->i_off[g + 1]; i++) {
             const double p[3] = {t->x[i], t->y[i], t->z[i]};
             ob.add(p, rs[i]);
         }
@@ -382,14 +409,13 @@ int sphk_tree_list_symmetry(sphk_tree *t, const double *h)
         }
     }
     int rc = build_lists(t, t->lists[1], false, [&](int g, std::vector<int32_t> &out, std::vector<int32_t> *) {
-        const Node &gn = t->node[t->group[g]];
         Box ob;
         ob.empty();
-        for (int i = gn.begin; i < gn.end; i++) {
+        for (int i = t->i_off[g]; i < t->i_off[g + 1]; i++) {
             const double p[3] = {t->x[i], t->y[i], t->z[i]};
             ob.add(p, h[i]);
         }
-        const Box &ib = gn.box;
+        const Box &ib = t->gbox[g];
         std::vector<int32_t> st{0};
         while (!st.empty()) {
             const Node &nd = t->node[st.back()];
@@ -417,8 +443,7 @@ int sphk_tree_list_long(sphk_tree *t, double theta)
     if (!t || !(theta >= 0.)) return set_error(SPHK_E_ARG, "long: bad argument");
     const double inv_t2 = theta > 0. ? 1. / (theta * theta) : std::numeric_limits<double>::infinity();
     int rc = build_lists(t, t->lists[2], true, [&](int g, std::vector<int32_t> &out, std::vector<int32_t> *sp) {
-        const Node &gn = t->node[t->group[g]];
-        const Box &ib = gn.box;
+        const Box &ib = t->gbox[g];
         std::vector<int32_t> st{0};
         while (!st.empty()) {
             const int32_t a = st.back();

@@ -16,7 +16,7 @@ are possible, and sets up parabolic encounters directly:
     K = 4 pi G (R/xi_1)^2 rho_c^(1-1/n) / (n+1),  P = K rho^(1+1/n),
     u = P / ((gamma-1) rho), gamma = 5/3 (the reference's ideal gas,
     P = rho u / 1.5, sph_yamauchi/run/hdr_hgas.hpp:339-341, 767-768).
-  * H_0 = eta kappa (m / rho_LE(r))^(1/3); alpha = 1, alpha_u = 0
+  * H_0 = min(eta kappa (m / rho_LE(r))^(1/3), R/4); alpha = 1, alpha_u = 0
     (polytropic_initial_condition/make_and_output_data.hpp:12-13).
   * Random numbers: SplitMix64 -> 53-bit doubles, seed = 1000*config + star,
     three draws per particle in the order (m, cos theta, phi).
@@ -113,14 +113,14 @@ def polytrope(n_index: float, N: int, M: float = 1.0, R: float = 1.0, seed: int 
     rho_c = M * le.xi1 / (4.0 * math.pi * R ** 3 * (-le.dtheta1))
     th = np.interp(xi, le.xi, le.theta)
     rho = rho_c * np.maximum(th, 0.0) ** n_index
-    rho_floor = rho_c * 1e-6                           # surface particles: avoid rho = 0
-    rho = np.maximum(rho, rho_floor)
+    rho = np.maximum(rho, rho_c * 1e-4)               # surface particles: avoid rho = 0
     alpha_len = R / le.xi1
     K = 4.0 * math.pi * alpha_len ** 2 * rho_c ** (1.0 - 1.0 / n_index) / (n_index + 1.0)
     P = K * rho ** (1.0 + 1.0 / n_index)
     uene = P / ((GAMMA - 1.0) * rho)
     m = np.full(N, M / N)
-    ksr = eta * KSRH * (m / rho) ** (1.0 / 3.0)
+    # initial H; capped at R/4 where the Lane-Emden density vanishes at the surface
+    ksr = np.minimum(eta * KSRH * (m / rho) ** (1.0 / 3.0), 0.25 * R)
     return dict(
         id=np.arange(id0, id0 + N, dtype=np.int32), istar=np.full(N, istar, np.int32),
         m=m, x=x, y=y, z=z, vx=np.zeros(N), vy=np.zeros(N), vz=np.zeros(N),

@@ -1,0 +1,268 @@
+"""Device-resident SPH step: the reference's calcSPHKernel and KDK loop with
+the particle state in HBM and every O(N) and O(N*list) pass on the GPU.
+
+Order of one force evaluation (sph_yamauchi/run/main.cpp:293-358):
+  density (h-iteration, grad-h, div/curl)  -> state dens/divv/rotv/grdh/ksr
+  EOS + Balsara                            (hdr_hgas.hpp:318-344, hdr_sph.hpp:79-89)
+  softening eps2, tree gravity, write-back (hdr_gravity.hpp:107-137, 346-423)
+  hydro EPI prep, hydro, write-back, acc sum, alpha-dot
+                                           (hdr_hydro.hpp, hdr_sph.hpp:95-97, hdr_hgas.hpp:750-762)
+The walk lists come from the host tree (tree.py) and are uploaded once per
+list build; they are inputs of the step, not part of it.
+
+The density smoothing-length retry loop (hdr_density.hpp:369-415) is in
+`converge_density`: particles whose H_3 exceeds their search radius get
+rs *= 1.1 and a fresh gather list, and keep the results of the pass in
+which they converged.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import check, lib
+from .device import FP64, MIXED, Context, Walks, to_device
+from .tree import HostTree, morton_order
+
+KSRH = 1.936492
+
+
+def phys(**kw) -> L.Phys:
+    d = dict(ninv=1.0 / 1.5, ksrh=KSRH, G=1.0, alpha_max=2.0, alpha_min=0.1, alphu_max=0.0,
+             alphu_min=0.0, eps_u=0.0, c_dt=0.1, dt_max=10.0)
+    d.update(kw)
+    return L.Phys(**d)
+
+
+class SPHSystem:
+    """Particle state on the device plus the current walk lists."""
+
+    def __init__(self, p: dict, ctx: Context, eta: float = 1.6, eps: float | None = None,
+                 theta: float = 0.5, n_group: int = 16, n_leaf: int = 8,
+                 precision: int = MIXED, hmax: float | None = None, phys_params=None,
+                 device="cuda"):
+        self.ctx = ctx
+        self.dev = device
+        self.eta = eta
+        self.eps = eps
+        self.theta = theta
+        self.n_group = n_group
+        self.n_leaf = n_leaf
+        self.precision = precision
+        self.phys = phys_params or phys()
+        perm = morton_order(p["x"], p["y"], p["z"])
+        self.n = n = len(perm)
+        f64 = dict(dtype=torch.float64, device=device)
+        self.s = {k: torch.zeros(n, **f64) for k in L.STATE_FIELDS}
+        for k in ("m", "x", "y", "z", "vx", "vy", "vz", "uene", "ksr", "alph", "alphu"):
+            self.s[k].copy_(torch.from_numpy(np.ascontiguousarray(p[k][perm], np.float64)))
+        self.id = to_device(np.ascontiguousarray(p["id"][perm], np.int32), device)
+        self.istar = to_device(np.ascontiguousarray(p.get("istar", np.zeros(n, np.int32))[perm],
+                                                    np.int32), device)
+        self.rs = torch.zeros(n, **f64)
+        self.np_ = torch.zeros(n, dtype=torch.int64, device=device)
+        self.itr = torch.zeros(n, dtype=torch.uint8, device=device)
+        self.eps2 = torch.zeros(n, **f64)
+        self.hprep = {k: torch.zeros(n, **f64) for k in ("hinv", "presc", "bswt_half", "thrm")}
+        self.gout = {k: torch.zeros(n, **f64) for k in L.GRAV_OUT}
+        self.hout = {k: torch.zeros(n, **f64) for k in L.HYDRO_OUT}
+        self.time = 0.0
+        self.dt = self.phys.dt_max
+        if hmax is None:
+            hmax = float(self.s["ksr"].max().item()) * 100.0
+        ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax)
+        self.lists = {}
+        self.list_build_s = 0.0
+        self._ss = L.State(**{k: self.s[k].data_ptr() for k in L.STATE_FIELDS})
+
+    # ------------------------------------------------------------------ lists
+    def _host(self, *keys):
+        return [self.s[k].cpu().numpy() for k in keys]
+
+    def resort(self):
+        """Re-establish Morton order after particles moved (permutes the state)."""
+        x, y, z = self._host("x", "y", "z")
+        perm = morton_order(x, y, z)
+        if np.array_equal(perm, np.arange(self.n, dtype=perm.dtype)):
+            return
+        pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
+        for k in L.STATE_FIELDS:
+            self.s[k].copy_(self.s[k][pt])
+        for t in (self.id, self.istar, self.rs, self.np_, self.itr, self.eps2):
+            t.copy_(t[pt])
+
+    def build_lists(self, kinds=("dens", "hydro", "grav")):
+        t0 = time.perf_counter()
+        x, y, z, m, ksr = self._host("x", "y", "z", "m", "ksr")
+        if "grav" in kinds:
+            self._softening()
+        eps2 = self.eps2.cpu().numpy()
+        tree = HostTree(x, y, z, m, eps2, self.n_leaf, self.n_group)
+        if "dens" in kinds:
+            rs = self.rs.cpu().numpy()
+            self.lists["dens"] = Walks(tree.i_off, *tree.gather(rs), device=self.dev)
+        if "hydro" in kinds:
+            self.lists["hydro"] = Walks(tree.i_off, *tree.symmetry(ksr), device=self.dev)
+        if "grav" in kinds:
+            j_off, j_idx, s_off, s_idx = tree.long(self.theta)
+            self.lists["grav"] = Walks(tree.i_off, j_off, j_idx, s_off, s_idx, device=self.dev)
+            self.mom = {k: to_device(v, self.dev) for k, v in tree.moments().items()}
+        tree.close()
+        torch.cuda.synchronize()
+        self.list_build_s = time.perf_counter() - t0
+
+    # ---------------------------------------------------------------- kernels
+    def _st(self):
+        return torch.cuda.current_stream().cuda_stream
+
+    def _softening(self):
+        check(lib().sphk_softening_dev(self.ctx.handle, self.n, self.s["ksr"].data_ptr(), KSRH,
+                                       0.0 if self.eps is None else float(self.eps),
+                                       self.eps2.data_ptr(), self._st()), "sphk_softening_dev")
+
+    def dens_src(self):
+        s = self.s
+        return dict(id=self.id, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"],
+                    vz=s["vz"], ksr=s["ksr"], rs=self.rs)
+
+    def density(self, out: dict | None = None, precision=None):
+        """One density functor pass over the current gather lists."""
+        s = self.s
+        src = self.dens_src()
+        if out is None:
+            out = dict(dens=s["dens"], divv=s["divv"], rotv=s["rotv"], grdh=s["grdh"],
+                       ksr=s["ksr"], np=self.np_, itr=self.itr)
+        self.ctx.density(self.lists["dens"], src, src, out,
+                         self.precision if precision is None else precision)
+        return out
+
+    def converge_density(self, expand: float = 1.1, max_rounds: int = 50):
+        """calcDensityKernel (hdr_density.hpp:369-415) with host list rebuilds.
+
+        Returns the number of rounds.  Raises if particles still need a larger
+        search radius after max_rounds (the reference loops without a bound).
+        """
+        n = self.n
+        s = self.s
+        self.rs.copy_(expand * s["ksr"])
+        done = torch.zeros(n, dtype=torch.bool, device=self.dev)
+        tmp = {k: torch.empty(n, dtype=torch.float64, device=self.dev)
+               for k in ("dens", "divv", "rotv", "grdh", "ksr")}
+        tmp["np"] = torch.empty(n, dtype=torch.int64, device=self.dev)
+        tmp["itr"] = torch.empty(n, dtype=torch.uint8, device=self.dev)
+        for rnd in range(1, max_rounds + 1):
+            self.build_lists(("dens",))
+            self.density(out=tmp)
+            newly = (~done) & (tmp["itr"] == 0)
+            for k in ("dens", "divv", "rotv", "grdh", "ksr"):
+                s[k].copy_(torch.where(newly, tmp[k], s[k]))
+            self.np_.copy_(torch.where(newly, tmp["np"], self.np_))
+            self.itr.copy_(torch.where(done, self.itr, tmp["itr"]))
+            done |= newly
+            retry = ~done
+            if not bool(retry.any()):
+                return rnd
+            self.rs.copy_(torch.where(retry, self.rs * expand, self.rs))
+        raise RuntimeError(f"density did not converge in {max_rounds} rounds")
+
+    def relax_h(self, passes: int = 3) -> int:
+        """Run the smoothing-length retry loop `passes` times, each starting
+        from the previous H, so H reaches the fixed point of the three-pass
+        iteration (set-up only; the reference starts from relaxed stars)."""
+        rounds = 0
+        for _ in range(passes):
+            rounds += self.converge_density()
+        return rounds
+
+    def eos_balsara(self):
+        check(lib().sphk_eos_balsara_dev(self.ctx.handle, self.n, self._ss, self.phys, self._st()),
+              "sphk_eos_balsara_dev")
+
+    def gravity(self, precision=None):
+        self._softening()
+        s = self.s
+        src = dict(m=s["m"], x=s["x"], y=s["y"], z=s["z"], eps2=self.eps2)
+        self.ctx.gravity(self.lists["grav"], src, src, self.gout, ps=self.mom,
+                         precision=self.precision if precision is None else precision)
+        go = L.GravOut(**{k: self.gout[k].data_ptr() for k in L.GRAV_OUT})
+        check(lib().sphk_gravity_writeback_dev(self.ctx.handle, self.n, self._ss, self.phys, go,
+                                               self._st()), "sphk_gravity_writeback_dev")
+
+    def hydro_src(self):
+        s, h = self.s, self.hprep
+        return dict(id=self.id, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"],
+                    vz=s["vz"], hinv=h["hinv"], pres=s["pres"], presc=h["presc"],
+                    bswt=h["bswt_half"], dens=s["dens"], vsnd=s["vsnd"], alph=s["alph"],
+                    alphu=s["alphu"], thrm=h["thrm"], eta=s["eta"])
+
+    def hydro(self, precision=None):
+        h = self.hprep
+        check(lib().sphk_hydro_prep_dev(self.ctx.handle, self.n, self._ss, h["hinv"].data_ptr(),
+                                        h["presc"].data_ptr(), h["bswt_half"].data_ptr(),
+                                        h["thrm"].data_ptr(), self._st()), "sphk_hydro_prep_dev")
+        src = self.hydro_src()
+        self.ctx.hydro(self.lists["hydro"], src, src, self.hout,
+                       self.precision if precision is None else precision)
+        ho = L.HydroOut(**{k: self.hout[k].data_ptr() for k in L.HYDRO_OUT})
+        check(lib().sphk_hydro_writeback_dev(self.ctx.handle, self.n, self._ss, self.phys, ho,
+                                             self._st()), "sphk_hydro_writeback_dev")
+
+    def force(self, precision=None):
+        """One force evaluation on resident lists (no list rebuild, no retry)."""
+        self.density(precision=precision)
+        self.eos_balsara()
+        self.gravity(precision)
+        self.hydro(precision)
+
+    # ------------------------------------------------------------ integrator
+    def diagnostics(self) -> dict:
+        out = (C.c_double * 6)()
+        check(lib().sphk_diagnostics(self.ctx.handle, self.n, self._ss, self.phys, out, self._st()),
+              "sphk_diagnostics")
+        return dict(energy=out[0], momentum=np.array(out[1:4]), dt_min=out[4], mass=out[5])
+
+    def next_dt(self, dt_min_particle: float, dt_floor: float = 1e-16) -> float:
+        """Power-of-two timestep adjustment of main.cpp:241-273."""
+        dtc = min(self.phys.dt_max, dt_min_particle)
+        dt = self.dt
+        if dt > dtc:
+            while dt > dtc:
+                dt *= 0.5
+                if dt <= dt_floor:
+                    raise FloatingPointError(f"timestep underflow at t={self.time}: {dtc}")
+        elif 2.0 * dt <= dtc and 2.0 * dt <= self.phys.dt_max:
+            dt2 = 2.0 * dt
+            if self.time - int(self.time / dt2) * dt2 == 0.0:
+                dt *= 2.0
+        return dt
+
+    def initialize(self):
+        """startSimulation's first force evaluation (hdr_hgas.hpp:1335-1378)."""
+        self.converge_density()
+        self.eos_balsara()
+        self.build_lists(("hydro", "grav"))
+        self.gravity()
+        self.hydro()
+
+    def kdk(self):
+        """One loopSimulation step (hdr_hgas.hpp:1499-1539) with a fixed list
+        rebuild per step.  Returns the diagnostics before the step."""
+        d = self.diagnostics()
+        self.dt = self.next_dt(d["dt_min"])
+        dt = self.dt
+        check(lib().sphk_predict_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
+              "sphk_predict_dev")
+        self.resort()
+        self.converge_density()
+        self.eos_balsara()
+        self.build_lists(("hydro", "grav"))
+        self.gravity()
+        self.hydro()
+        check(lib().sphk_correct_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
+              "sphk_correct_dev")
+        self.time += dt
+        return d

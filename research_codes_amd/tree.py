@@ -50,6 +50,13 @@ class HostTree:
         ioff = lib().sphk_tree_i_off(h)
         self.i_off = np.ctypeslib.as_array(ioff, shape=(self.n_walk + 1,)).copy()
 
+    def group_fixed(self, size: int):
+        """Walks = consecutive Morton chunks of `size` particles."""
+        check(lib().sphk_tree_group_fixed(self._h, int(size)), "sphk_tree_group_fixed")
+        self.n_walk = lib().sphk_tree_n_walk(self._h)
+        ioff = lib().sphk_tree_i_off(self._h)
+        self.i_off = np.ctypeslib.as_array(ioff, shape=(self.n_walk + 1,)).copy()
+
     def close(self):
         if self._h:
             lib().sphk_tree_free(self._h)

@@ -28,7 +28,8 @@ def _dev(d, keys=None):
     return {k: D.to_device(v) for k, v in d.items() if keys is None or k in keys}
 
 
-@pytest.fixture(scope="module", params=[(2000, 64), (3000, 8)], ids=["N2000_g64", "N3000_g8"])
+@pytest.fixture(scope="module", params=[(2000, 16), (3000, 64), (4000, 256)],
+                ids=["N2000_g16", "N3000_g64", "N4000_g256"])
 def case(request):
     N, g = request.param
     print(f"[parity] building case N={N} n_group={g}", flush=True)
