@@ -1,0 +1,127 @@
+"""CPU-side checks of the product library: the C-ABI exports what include/*.h
+declares, host-side argument validation returns error codes (no GPU needed),
+and the host tree produces walks and lists that satisfy their policies
+(include/sphk_tree.h) against brute force.
+"""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from research_codes_amd import _lib as L
+from research_codes_amd import ic
+from research_codes_amd.tree import HostTree, morton_order, sort_particles
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_c_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"\b(sphk_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    declared = _declared_c_functions()
+    assert len(declared) > 30
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    # the ctypes table covers the same set
+    assert declared <= set(L.SIGNATURES), declared - set(L.SIGNATURES)
+
+
+def test_host_side_validation_without_gpu():
+    lib = L.lib()
+    p = L.Params(1.6, 1.936492, 1.0)
+    assert lib.sphk_set_params(None, C.byref(p)) == -1
+    assert b"null" in lib.sphk_last_error()
+    assert lib.sphk_density_dev(None, 1, None, None, None, L.DensSrc(), L.DensSrc(), L.DensOut(),
+                                0, None) == -1
+    assert lib.sphk_synchronize(None) == -1
+    assert lib.sphk_kernel_time(None, 0, None, None) == -1
+    assert lib.sphk_finalize(None) == 0
+
+
+def test_morton_order_is_a_sorted_permutation():
+    rng = np.random.default_rng(0)
+    x, y, z = rng.uniform(-1, 1, (3, 5000))
+    perm = morton_order(x, y, z)
+    assert np.array_equal(np.sort(perm), np.arange(5000))
+    # building a tree on the permuted particles accepts them as sorted
+    t = HostTree(x[perm], y[perm], z[perm], np.ones(5000))
+    assert t.i_off[0] == 0 and t.i_off[-1] == 5000 and np.all(np.diff(t.i_off) > 0)
+    with pytest.raises(L.SphkError):
+        HostTree(x, y, z, np.ones(5000))     # unsorted input is rejected
+    t.close()
+
+
+@pytest.fixture(scope="module")
+def star():
+    p = ic.polytrope(1.5, 4000, seed=9, eta=1.6)
+    p, _ = sort_particles(p)
+    return p
+
+
+def _walk_of(i_off, n):
+    w = np.zeros(n, np.int64)
+    for k in range(len(i_off) - 1):
+        w[i_off[k]:i_off[k + 1]] = k
+    return w
+
+
+@pytest.mark.parametrize("n_group", [8, 16, 64, 256])
+def test_gather_and_symmetry_lists_are_supersets(star, n_group):
+    p = star
+    n = len(p["m"])
+    t = HostTree(p["x"], p["y"], p["z"], p["m"], None, 8, n_group)
+    assert np.diff(t.i_off).max() <= max(n_group, 8)
+    X = np.stack([p["x"], p["y"], p["z"]], 1)
+    r = np.sqrt(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1))
+    rs = 1.1 * p["ksr"]
+    jo, ji = t.gather(rs)
+    h = p["ksr"]
+    so, si = t.symmetry(h)
+    for w in range(t.n_walk):
+        ii = np.arange(t.i_off[w], t.i_off[w + 1])
+        glist = set(ji[jo[w]:jo[w + 1]].tolist())
+        need = set(np.nonzero((r[ii] < rs[ii, None]).any(0))[0].tolist())
+        assert need <= glist
+        slist = set(si[so[w]:so[w + 1]].tolist())
+        need_s = set(np.nonzero((r[ii] < np.maximum(h[ii, None], h[None, :])).any(0))[0].tolist())
+        assert need_s <= slist
+        assert len(glist) == jo[w + 1] - jo[w]          # no duplicates
+    t.close()
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.8])
+def test_long_lists_cover_every_particle_once(star, theta):
+    p = star
+    n = len(p["m"])
+    t = HostTree(p["x"], p["y"], p["z"], p["m"], np.full(n, 1e-4), 8, 16)
+    jo, ji, so, si = t.long(theta)
+    mom = t.moments()
+    M = p["m"].sum()
+    assert abs(mom["m"][0] / M - 1) < 1e-13
+    for c in ("x", "y", "z"):
+        assert abs(mom[c][0] - (p["m"] * p[c]).sum() / M) < 1e-13
+    for w in range(0, t.n_walk, 7):
+        m_cover = p["m"][ji[jo[w]:jo[w + 1]]].sum() + mom["m"][si[so[w]:so[w + 1]]].sum()
+        assert abs(m_cover / M - 1) < 1e-12
+        own = np.arange(t.i_off[w], t.i_off[w + 1])
+        assert set(own.tolist()) <= set(ji[jo[w]:jo[w + 1]].tolist())
+    t.close()
+
+
+def test_fixed_groups_partition(star):
+    p = star
+    t = HostTree(p["x"], p["y"], p["z"], p["m"])
+    t.group_fixed(32)
+    assert np.array_equal(t.i_off[:-1], np.arange(0, len(p["m"]), 32))
+    assert t.i_off[-1] == len(p["m"])
+    t.close()
