@@ -20,8 +20,10 @@
  *    list, as the reference's Dispatch builds, hdr_densityGPU.cu:513-566).
  *  - Arrays are structure-of-arrays, FP64, ids int32.
  *  - "_dev" entry points take DEVICE pointers (inputs resident in HBM) and
- *    launch asynchronously on `stream` (a hipStream_t passed as void*, NULL
- *    for the context's own stream).  "_dispatch"/"_retrieve" take HOST
+ *    launch asynchronously on `stream`, a hipStream_t passed as void*; NULL
+ *    is the HIP null stream (the legacy default stream, which is where a
+ *    caller's default stream -- torch's included -- runs).  Pass
+ *    sphk_stream(ctx) to use the context's own (non-blocking) stream.  "_dispatch"/"_retrieve" take HOST
  *    pointers and mirror the reference's Dispatch/Retrieve split: dispatch
  *    copies in and launches, retrieve waits and copies out.
  *  - precision: SPHK_FP64 evaluates every pair term in FP64; SPHK_MIXED

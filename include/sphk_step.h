@@ -1,6 +1,7 @@
 /*
  * sphk_step.h -- per-particle step kernels around the interaction functors
- * (C-ABI of libsphk.so; device pointers, asynchronous on `stream`).
+ * (C-ABI of libsphk.so; device pointers, asynchronous on `stream`, a
+ * hipStream_t as void*, NULL being the HIP null stream as in sphk.h).
  *
  * These are the reference's O(N) host loops between and after the three
  * force passes of calcSPHKernel (sph_yamauchi/run/main.cpp:293-358) and its

@@ -81,8 +81,9 @@ __device__ __forceinline__ void slice_reduce(double (&v)[NQ], double *red, const
 // few ulps of the kernel edge.
 __device__ __attribute__((noinline)) double exact_q(double dx, double dy, double dz, double hinv)
 {
-    const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-    return __dmul_rn(__builtin_sqrt(r2), hinv);
+#pragma clang fp contract(off)
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    return __builtin_sqrt(r2) * hinv;
 }
 
 struct Pass1 {

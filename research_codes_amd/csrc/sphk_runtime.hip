@@ -180,10 +180,9 @@ struct Timer {
     }
 };
 
-hipStream_t pick_stream(sphk_ctx *ctx, void *stream)
-{
-    return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
-}
+// Device-resident launches run on exactly the stream the caller names; NULL is
+// the HIP null stream (what a caller's default stream, e.g. torch's, maps to).
+hipStream_t pick_stream(sphk_ctx *, void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 
 int check_ctx(sphk_ctx *ctx)
 {
@@ -497,7 +496,7 @@ int sphk_density_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip
     slot.ni_tot = ni;
     return sphk_density_dev(ctx, n_walk, (const int32_t *)(slot.din.p + items[0].off),
                             (const int32_t *)(slot.din.p + items[1].off), nullptr, di, dj, dout,
-                            precision, nullptr);
+                            precision, (void *)ctx->stream);
 }
 
 int sphk_density_retrieve(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip, sphk_dens_out out)
@@ -535,7 +534,7 @@ int sphk_hydro_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip, 
     slot.ni_tot = ni;
     return sphk_hydro_dev(ctx, n_walk, (const int32_t *)(slot.din.p + items[0].off),
                           (const int32_t *)(slot.din.p + items[1].off), nullptr, di, dj, dout,
-                          precision, nullptr);
+                          precision, (void *)ctx->stream);
 }
 
 int sphk_hydro_retrieve(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip, sphk_hydro_out out)
@@ -557,6 +556,7 @@ int sphk_gravity_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip
     if ((rc = prefix(n_walk, nip, ioff, SPHK_NI_LIMIT, "nip"))) return rc;
     if ((rc = prefix(n_walk, njp, joff, SPHK_NJ_LIMIT, "njp"))) return rc;
     if (nsp && (rc = prefix(n_walk, nsp, soff, SPHK_NJ_LIMIT, "nsp"))) return rc;
+    if (nsp && soff[n_walk] == 0) nsp = nullptr;  // no SPJ at all: same as no list
     const size_t ni = ioff[n_walk], nj = joff[n_walk], ns = nsp ? soff[n_walk] : 0;
     if ((size_t)nj + ns > (size_t)SPHK_NJ_LIMIT)
         return set_error(SPHK_E_CAPACITY, "sum(njp)+sum(nsp) exceeds %d", SPHK_NJ_LIMIT);
@@ -582,7 +582,7 @@ int sphk_gravity_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip
     return sphk_gravity_dev(ctx, n_walk, (const int32_t *)(slot.din.p + items[0].off),
                             (const int32_t *)(slot.din.p + items[1].off), nullptr, dj,
                             nsp ? (const int32_t *)(slot.din.p + items[2].off) : nullptr, nullptr,
-                            ds, di, dout, precision, nullptr);
+                            ds, di, dout, precision, (void *)ctx->stream);
 }
 
 int sphk_gravity_retrieve(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip, sphk_grav_out out)

@@ -224,11 +224,8 @@ using namespace sphk;
 
 namespace {
 
-hipStream_t stream_of(sphk_ctx *ctx, void *stream)
-{
-    return stream ? reinterpret_cast<hipStream_t>(stream)
-                  : reinterpret_cast<hipStream_t>(sphk_stream(ctx));
-}
+// The caller's stream; NULL is the HIP null stream (see include/sphk.h).
+hipStream_t stream_of(sphk_ctx *, void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 
 int launch_check(const char *what)
 {

@@ -73,11 +73,37 @@ class SPHSystem:
         self.time = 0.0
         self.dt = self.phys.dt_max
         if hmax is None:
-            hmax = float(self.s["ksr"].max().item()) * 100.0
+            hmax = self.system_size()
         ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax)
         self.lists = {}
         self.list_build_s = 0.0
         self._ss = L.State(**{k: self.s[k].data_ptr() for k in L.STATE_FIELDS})
+
+    # ---------------------------------------------------------- H_max (glue)
+    def _com(self, star: int):
+        sel = (self.istar == star).to(torch.float64)
+        m = self.s["m"] * sel
+        M = m.sum()
+        return M, [(m * self.s[c]).sum() / M for c in ("x", "y", "z")]
+
+    def system_size(self) -> float:
+        """calcSystemSize (hdr_util.hpp:93-113, FlagBinary 0): distance between
+        the centres of mass of star 0 and star 1; unbounded for one star."""
+        M0, c0 = self._com(0)
+        M1, c1 = self._com(1)
+        if float(M0) == 0.0 or float(M1) == 0.0:
+            return 1e30
+        return float(torch.sqrt(sum((a - b) ** 2 for a, b in zip(c0, c1))))
+
+    def update_hmax(self) -> float:
+        """calcKernelSupportRadiusMaximum (hdr_util.hpp:489-505): density-
+        weighted RMS distance of all particles from star 0's centre of mass."""
+        _, c0 = self._com(0)
+        d = self.s["dens"]
+        r2 = sum((self.s[c] - x0) ** 2 for c, x0 in zip(("x", "y", "z"), c0))
+        h = float(torch.sqrt((d * r2).sum() / d.sum()))
+        self.ctx.set_params(hmax=h)
+        return h
 
     # ------------------------------------------------------------------ lists
     def _host(self, *keys):
@@ -140,11 +166,15 @@ class SPHSystem:
                          self.precision if precision is None else precision)
         return out
 
-    def converge_density(self, expand: float = 1.1, max_rounds: int = 50):
+    def converge_density(self, expand: float = 1.1, max_rounds: int = 100, jump: bool = True):
         """calcDensityKernel (hdr_density.hpp:369-415) with host list rebuilds.
 
-        Returns the number of rounds.  Raises if particles still need a larger
-        search radius after max_rounds (the reference loops without a bound).
+        A particle whose H_3 exceeds its search radius retries with
+        rs *= 1.1 (the reference's schedule, jump=False) or with
+        rs = max(1.1 rs, 1.1 H_3) (jump=True, default: the same fixed point in
+        fewer rounds for particles far from their neighbours).  Returns the
+        number of rounds; raises if particles still need a larger search
+        radius after max_rounds (the reference loops without a bound).
         """
         n = self.n
         s = self.s
@@ -166,7 +196,10 @@ class SPHSystem:
             retry = ~done
             if not bool(retry.any()):
                 return rnd
-            self.rs.copy_(torch.where(retry, self.rs * expand, self.rs))
+            grown = self.rs * expand
+            if jump:
+                grown = torch.maximum(grown, expand * tmp["ksr"])
+            self.rs.copy_(torch.where(retry, grown, self.rs))
         raise RuntimeError(f"density did not converge in {max_rounds} rounds")
 
     def relax_h(self, passes: int = 3) -> int:
@@ -247,6 +280,7 @@ class SPHSystem:
         self.build_lists(("hydro", "grav"))
         self.gravity()
         self.hydro()
+        self.update_hmax()
 
     def kdk(self):
         """One loopSimulation step (hdr_hgas.hpp:1499-1539) with a fixed list
@@ -262,6 +296,7 @@ class SPHSystem:
         self.build_lists(("hydro", "grav"))
         self.gravity()
         self.hydro()
+        self.update_hmax()
         check(lib().sphk_correct_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_correct_dev")
         self.time += dt
