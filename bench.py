@@ -63,6 +63,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="per functor")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run 3 plain 256 MiB device copies (counter scale reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU spatial decomposition "
@@ -76,6 +78,14 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)          # reads and writes exactly 256 MiB
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 3 x 256 MiB device-to-device copies")
 
     t0 = time.perf_counter()
     p = ic.off_axis(args.n, eta=args.eta)

@@ -71,6 +71,11 @@ typedef struct {
     double eta;   /* SK::eta, 1.6 in the reference */
     double ksrh;  /* SK::ksrh (kappa), 1.936492 */
     double hmax;  /* RP::KernelSupportRadiusMaximum */
+    /* Nonzero: the caller guarantees that every eps2 the gravity functor sees
+     * (EPI, EPJ and SPJ) is the same value, e.g. a constant gravitational
+     * softening.  The symmetrised pair term then needs one rsqrt instead of
+     * two; results are those of the general formula.  0 (default): general. */
+    int uniform_softening;
 } sphk_params;
 
 /* ---- inputs / outputs (all FP64 unless stated) ------------------------ */

@@ -9,12 +9,12 @@
  * (hdr_density.hpp:389, main.cpp:339, hdr_gravity.hpp:393).  This library
  * provides its own octree with the same roles; the list policies are:
  *
- *  - gather (density): j is listed for a group iff x_j lies in the group's
- *    outer box, the bounding box of the spheres |x - x_i| < rs_i of its
- *    i-particles.
- *  - symmetry (hydro): j is listed iff x_j lies in the outer box built with
- *    H_i, or the box x_j +- H_j overlaps the bounding box of the group's
- *    positions (a superset of all pairs with r_ij < max(H_i, H_j)).
+ *  - gather (density): j is listed for a group iff |x_j - x_i| < rs_i for at
+ *    least one i-particle of the group (the union of the i search spheres;
+ *    the tree walk prunes with the bounding box of those spheres).
+ *  - symmetry (hydro): j is listed iff r_ij < max(H_i, H_j) for at least one
+ *    i of the group (pruned with the box of the i-spheres and the boxes
+ *    x_j +- H_j).
  *  - long (gravity): a node is accepted as one SPJ when the squared distance
  *    from its centre of mass to the group's bounding box exceeds
  *    (L_node / theta)^2, L_node the node's cubic cell side; otherwise it is

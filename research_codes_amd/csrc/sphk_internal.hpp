@@ -19,7 +19,7 @@ hipError_t launch_hydro(hipStream_t st, int n_walk, const int32_t *i_off, const 
 hipError_t launch_gravity(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_grav_src &pj, const int32_t *s_off,
                           const int32_t *s_idx, const sphk_grav_src &ps, const sphk_grav_src &pi,
-                          const sphk_grav_out &out, bool mixed);
+                          const sphk_grav_out &out, bool mixed, bool uniform);
 
 // Records the error text returned by sphk_last_error() and passes the code through.
 int set_error(int code, const char *fmt, ...);

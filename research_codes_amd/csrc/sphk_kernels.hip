@@ -442,25 +442,42 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
 // Gravity: scalar calcGravity<TJ>, sph_yamauchi/run/hdr_gravity.hpp:295-338,
 // over the EPJ list and then the SPJ list of each walk.
 // ---------------------------------------------------------------------------
+// j-record staged in LDS: position FP64; mass and softening in the pair
+// precision (one 32-byte record in the mixed path).
+template <bool MIXED>
+struct GRec;
+template <>
+struct alignas(16) GRec<true> {
+    double x, y, z;
+    float m, e;
+};
+template <>
+struct alignas(8) GRec<false> {
+    double x, y, z, m, e;
+};
+
 template <typename R>
 struct GravAcc {
     R ax, ay, az, pt, et;
 };
 
-template <typename R>
-__device__ __forceinline__ void gravity_pair(const double4 &p, R e2j, double px, double py,
-                                             double pz, R e2i, GravAcc<R> &a)
+// General pair term (softening of i and of j may differ):
+// acc -= m x (rei3_i + rei3_j), pot -= m (rei_i + rei_j), eta += m rei3_i;
+// the per-particle totals are halved at the end (hdr_gravity.hpp:310-331).
+template <bool MIXED>
+__device__ __forceinline__ void gravity_pair(const GRec<MIXED> &p, double px, double py, double pz,
+                                             real_t<MIXED> e2i, GravAcc<real_t<MIXED>> &a)
 {
-    // p = {x, y, z, m}
+    using R = real_t<MIXED>;
     const R dpx = (R)(px - p.x);
     const R dpy = (R)(py - p.y);
     const R dpz = (R)(pz - p.z);
     const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
     const R rei_i = rsqrt_r(r2 + e2i);
-    const R rei_j = rsqrt_r(r2 + e2j);
+    const R rei_j = rsqrt_r(r2 + (R)p.e);
     const R rei3_i = rei_i * rei_i * rei_i;
     const R rei3_j = rei_j * rei_j * rei_j;
-    const R m_j = (R)p.w;
+    const R m_j = (R)p.m;
     const R dg2 = m_j * (rei3_i + rei3_j);
     a.pt -= m_j * (rei_i + rei_j);
     a.ax -= dpx * dg2;
@@ -469,11 +486,35 @@ __device__ __forceinline__ void gravity_pair(const double4 &p, R e2j, double px,
     a.et += m_j * rei3_i;
 }
 
+// Uniform softening (every eps2 equal, e.g. a constant gravitational
+// softening): the two symmetrised terms coincide, one rsqrt per pair.
+// Accumulates +m x rei3, +m rei, +m rei3; the caller finishes with the sign.
 template <bool MIXED>
+__device__ __forceinline__ void gravity_pair_uniform(const GRec<MIXED> &p, double px, double py,
+                                                     double pz, real_t<MIXED> e2,
+                                                     GravAcc<real_t<MIXED>> &a)
+{
+    using R = real_t<MIXED>;
+    const R dpx = (R)(px - p.x);
+    const R dpy = (R)(py - p.y);
+    const R dpz = (R)(pz - p.z);
+    const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+    const R rei = rsqrt_r(r2 + e2);
+    const R m_j = (R)p.m;
+    const R mr = m_j * rei;
+    const R w = mr * rei * rei;
+    a.pt += mr;
+    a.ax += dpx * w;
+    a.ay += dpy * w;
+    a.az += dpz * w;
+    a.et += w;
+}
+
+template <bool MIXED, bool UNIFORM>
 __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int ke,
                                              const int32_t *__restrict__ idx, sphk_grav_src src,
                                              double px, double py, double pz, double e2_i,
-                                             double4 *sp, real_t<MIXED> *se, double (&acc)[5])
+                                             GRec<MIXED> *sp, double (&acc)[5])
 {
     using R = real_t<MIXED>;
     const R e2 = (R)e2_i;
@@ -482,24 +523,33 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
         __syncthreads();
         if (lane < nt) {
             const int j = idx ? idx[t0 + lane] : t0 + lane;
-            sp[lane] = make_double4(src.x[j], src.y[j], src.z[j], src.m[j]);
-            se[lane] = (R)src.eps2[j];
+            sp[lane] = GRec<MIXED>{src.x[j], src.y[j], src.z[j], (R)src.m[j], (R)src.eps2[j]};
         } else {
-            sp[lane] = make_double4(kFar, kFar, kFar, 0.);
-            se[lane] = R(1);
+            sp[lane] = GRec<MIXED>{kFar, kFar, kFar, R(0), R(1)};
         }
         __syncthreads();
         GravAcc<R> a{R(0), R(0), R(0), R(0), R(0)};
         int k = M.slice;
         for (; k + 3 * M.S < kTile; k += 4 * M.S) {
-            const double4 p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
-            const R e0 = se[k], e1 = se[k + M.S], e2_ = se[k + 2 * M.S], e3 = se[k + 3 * M.S];
-            gravity_pair<R>(p0, e0, px, py, pz, e2, a);
-            gravity_pair<R>(p1, e1, px, py, pz, e2, a);
-            gravity_pair<R>(p2, e2_, px, py, pz, e2, a);
-            gravity_pair<R>(p3, e3, px, py, pz, e2, a);
+            const GRec<MIXED> p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
+            if (UNIFORM) {
+                gravity_pair_uniform<MIXED>(p0, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED>(p1, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED>(p2, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED>(p3, px, py, pz, e2, a);
+            } else {
+                gravity_pair<MIXED>(p0, px, py, pz, e2, a);
+                gravity_pair<MIXED>(p1, px, py, pz, e2, a);
+                gravity_pair<MIXED>(p2, px, py, pz, e2, a);
+                gravity_pair<MIXED>(p3, px, py, pz, e2, a);
+            }
         }
-        for (; k < kTile; k += M.S) gravity_pair<R>(sp[k], se[k], px, py, pz, e2, a);
+        for (; k < kTile; k += M.S) {
+            if (UNIFORM)
+                gravity_pair_uniform<MIXED>(sp[k], px, py, pz, e2, a);
+            else
+                gravity_pair<MIXED>(sp[k], px, py, pz, e2, a);
+        }
         acc[0] += (double)a.ax;
         acc[1] += (double)a.ay;
         acc[2] += (double)a.az;
@@ -508,18 +558,16 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
     }
 }
 
-template <bool MIXED>
+template <bool MIXED, bool UNIFORM>
 __global__ __launch_bounds__(kWave) void gravity_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_grav_src pj, const int32_t *__restrict__ s_off,
     const int32_t *__restrict__ s_idx, sphk_grav_src ps, sphk_grav_src pi, sphk_grav_out out)
 {
-    using R = real_t<MIXED>;
-    constexpr int kTileBytes = kTile * (sizeof(double4) + sizeof(R));
+    constexpr int kTileBytes = kTile * sizeof(GRec<MIXED>);
     constexpr int kRedBytes = 5 * kWave * sizeof(double);
     __shared__ __attribute__((aligned(32))) char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
-    double4 *sp = reinterpret_cast<double4 *>(smem);
-    R *se = reinterpret_cast<R *>(sp + kTile);
+    GRec<MIXED> *sp = reinterpret_cast<GRec<MIXED> *>(smem);
     double *red = reinterpret_cast<double *>(smem);
 
     const int w = blockIdx.x;
@@ -532,16 +580,25 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
         const int i = c0 + M.il;
         const double px = pi.x[i], py = pi.y[i], pz = pi.z[i], e2 = pi.eps2[i];
         double acc[5] = {0., 0., 0., 0., 0.};
-        gravity_list<MIXED>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp, se, acc);
+        gravity_list<MIXED, UNIFORM>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
+                                     acc);
         if (s_off)
-            gravity_list<MIXED>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2, sp, se,
-                                acc);
+            gravity_list<MIXED, UNIFORM>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
+                                         sp, acc);
         slice_reduce<5>(acc, red, M, lane);
         if (M.lead) {
-            out.ax[i] = acc[0] * 0.5;
-            out.ay[i] = acc[1] * 0.5;
-            out.az[i] = acc[2] * 0.5;
-            out.pot[i] = acc[3] * 0.5;
+            if (UNIFORM) {
+                // 0.5 * (-2 sum m rei3 x) etc.
+                out.ax[i] = -acc[0];
+                out.ay[i] = -acc[1];
+                out.az[i] = -acc[2];
+                out.pot[i] = -acc[3];
+            } else {
+                out.ax[i] = acc[0] * 0.5;
+                out.ay[i] = acc[1] * 0.5;
+                out.az[i] = acc[2] * 0.5;
+                out.pot[i] = acc[3] * 0.5;
+            }
             out.eta[i] = acc[4];
         }
     }
@@ -581,15 +638,17 @@ hipError_t launch_hydro(hipStream_t st, int n_walk, const int32_t *i_off, const 
 hipError_t launch_gravity(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_grav_src &pj, const int32_t *s_off,
                           const int32_t *s_idx, const sphk_grav_src &ps, const sphk_grav_src &pi,
-                          const sphk_grav_out &out, bool mixed)
+                          const sphk_grav_out &out, bool mixed, bool uniform)
 {
     if (n_walk <= 0) return hipSuccess;
-    if (mixed)
-        hipLaunchKernelGGL(gravity_kernel<true>, dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,
-                           j_off, j_idx, pj, s_off, s_idx, ps, pi, out);
-    else
-        hipLaunchKernelGGL(gravity_kernel<false>, dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,
-                           j_off, j_idx, pj, s_off, s_idx, ps, pi, out);
+#define SPHK_LAUNCH_G(M, U)                                                                         \
+    hipLaunchKernelGGL((gravity_kernel<M, U>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, \
+                       j_idx, pj, s_off, s_idx, ps, pi, out)
+    if (mixed && uniform) SPHK_LAUNCH_G(true, true);
+    else if (mixed) SPHK_LAUNCH_G(true, false);
+    else if (uniform) SPHK_LAUNCH_G(false, true);
+    else SPHK_LAUNCH_G(false, false);
+#undef SPHK_LAUNCH_G
     return hipGetLastError();
 }
 

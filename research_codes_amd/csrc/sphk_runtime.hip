@@ -136,7 +136,7 @@ struct TimeRec {
 struct sphk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    sphk_params params{1.6, 1.936492, 1e30};
+    sphk_params params{1.6, 1.936492, 1e30, 0};
     sphk::TagSlot tags[SPHK_TAG_MAX];
     bool timing = false;
     std::vector<sphk::TimeRec> pending;
@@ -459,7 +459,7 @@ int sphk_gravity_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
     hipStream_t st = pick_stream(ctx, stream);
     Timer t(ctx, 2, st);
     SPHK_HIP(launch_gravity(st, n_walk, i_off, j_off, j_idx, pj, s_off, s_idx, ps, pi, out,
-                            precision == SPHK_MIXED));
+                            precision == SPHK_MIXED, ctx->params.uniform_softening != 0));
     t.stop();
     return SPHK_OK;
 }

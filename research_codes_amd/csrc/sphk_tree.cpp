@@ -381,7 +381,15 @@ int sphk_tree_list_gather(sphk_tree *t, const double *rs)
             if (nd.child < 0) {
                 for (int j = nd.begin; j < nd.end; j++) {
                     const double p[3] = {t->x[j], t->y[j], t->z[j]};
-                    if (ob.contains(p)) out.push_back(j);
+                    if (!ob.contains(p)) continue;
+                    // exact membership: j lies in the search sphere of some i
+                    for (int i = t->i_off[g]; i < t->i_off[g + 1]; i++) {
+                        const double dx = p[0] - t->x[i], dy = p[1] - t->y[i], dz = p[2] - t->z[i];
+                        if (dx * dx + dy * dy + dz * dz < rs[i] * rs[i]) {
+                            out.push_back(j);
+                            break;
+                        }
+                    }
                 }
             } else {
                 for (int c = nd.child + nd.nchild - 1; c >= nd.child; c--) st.push_back(c);
@@ -427,7 +435,16 @@ int sphk_tree_list_symmetry(sphk_tree *t, const double *h)
                     Box jb;
                     jb.empty();
                     jb.add(p, h[j]);
-                    if (ob.contains(p) || jb.overlaps(ib)) out.push_back(j);
+                    if (!ob.contains(p) && !jb.overlaps(ib)) continue;
+                    // exact membership: r_ij < max(H_i, H_j) for some i
+                    for (int i = t->i_off[g]; i < t->i_off[g + 1]; i++) {
+                        const double dx = p[0] - t->x[i], dy = p[1] - t->y[i], dz = p[2] - t->z[i];
+                        const double hm = h[i] > h[j] ? h[i] : h[j];
+                        if (dx * dx + dy * dy + dz * dz < hm * hm) {
+                            out.push_back(j);
+                            break;
+                        }
+                    }
                 }
             } else {
                 for (int c = nd.child + nd.nchild - 1; c >= nd.child; c--) st.push_back(c);

@@ -103,7 +103,7 @@ class Context:
         except Exception:
             pass
 
-    def set_params(self, eta=None, ksrh=None, hmax=None):
+    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None):
         p = L.Params()
         if getattr(self, "_params", None) is not None:
             p = self._params
@@ -113,12 +113,15 @@ class Context:
             p.ksrh = float(ksrh)
         if hmax is not None:
             p.hmax = float(hmax)
+        if uniform_softening is not None:
+            p.uniform_softening = int(bool(uniform_softening))
         check(lib().sphk_set_params(self._h, C.byref(p)), "sphk_set_params")
         self._params = p
 
     @property
     def params(self):
-        return dict(eta=self._params.eta, ksrh=self._params.ksrh, hmax=self._params.hmax)
+        return dict(eta=self._params.eta, ksrh=self._params.ksrh, hmax=self._params.hmax,
+                    uniform_softening=bool(self._params.uniform_softening))
 
     @staticmethod
     def _stream():

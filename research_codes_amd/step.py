@@ -74,7 +74,8 @@ class SPHSystem:
         self.dt = self.phys.dt_max
         if hmax is None:
             hmax = self.system_size()
-        ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax)
+        # a constant softening makes every eps2 equal: one rsqrt per gravity pair
+        ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax, uniform_softening=eps is not None)
         self.lists = {}
         self.list_build_s = 0.0
         self._ss = L.State(**{k: self.s[k].data_ptr() for k in L.STATE_FIELDS})

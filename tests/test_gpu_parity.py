@@ -99,3 +99,27 @@ def test_gravity(ctx, case, prec):
         e = scaled_err(got[k], ref[k], floor_frac=1e-1 if k in ("ax", "ay", "az") else 1e-3)
         print(f"gravity[{prec}] {k}: {e:.3e}")
         assert e < tol, k
+
+
+@pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
+def test_gravity_uniform_softening(ctx, prec):
+    """Constant softening: the one-rsqrt path (uniform_softening) must equal
+    the oracle's general symmetrised formula."""
+    case = polytrope_case(2500, seed=31, n_group=16, eps=0.01)
+    jl = case["grav_lists"]
+    w = D.Walks(case["grav_i_off"], jl[0], jl[1], jl[2], jl[3])
+    src = _dev(case["grav_src"])
+    mom = _dev(case["grav_mom"])
+    out = D.empty_out(D.L.GRAV_OUT, len(case["p"]["m"]))
+    ctx.set_params(uniform_softening=True)
+    try:
+        ctx.gravity(w, src, src, out, ps=mom, precision=prec)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_params(uniform_softening=False)
+    ref = case["grav_out"]
+    tol = RTOL_FP64 if prec == D.FP64 else RTOL_MIXED
+    for k in D.L.GRAV_OUT:
+        e = scaled_err(out[k].cpu().numpy(), ref[k], floor_frac=1e-1 if k in ("ax", "ay", "az") else 1e-3)
+        print(f"gravity-uniform[{prec}] {k}: {e:.3e}")
+        assert e < tol, k
