@@ -63,6 +63,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="per functor")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="use --n-group for all kernels instead of tuning 8/16/32 per kernel")
+    ap.add_argument("--groups", default=None,
+                    help="walk group sizes dens,hydro,grav (e.g. 16,16,32); implies --no-autotune")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run 3 plain 256 MiB device copies (counter scale reference)")
     args = ap.parse_args()
@@ -92,23 +96,39 @@ def main() -> int:
     log(f"ic N={args.n} in {time.perf_counter() - t0:.1f}s")
     ctx = Context(0)
     ctx.timing(True)
-    sysm = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=args.n_group,
+    n_group = args.n_group
+    if args.groups:
+        g = [int(v) for v in args.groups.split(",")]
+        n_group = dict(zip(("dens", "hydro", "grav"), g))
+        args.autotune = False
+    sysm = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=n_group,
                      precision=MIXED)
     t0 = time.perf_counter()
     rounds = sysm.relax_h(3)
     sysm.eos_balsara()
     log(f"density converged in {rounds} round(s), {time.perf_counter() - t0:.1f}s")
     sysm.rs.copy_(1.1 * sysm.s["ksr"])
-    sysm.build_lists(("dens", "hydro", "grav"))
+    sysm.build_lists(("dens",))
     list_s = sysm.list_build_s
+    snap = {k: v.clone() for k, v in sysm.s.items()}
+    # every timed step starts from `snap`, so its density pass gives this H:
+    # the symmetry lists are built from it, as the reference builds them
+    # after calcDensityKernel
+    sysm.density()
+    sysm.eos_balsara()
+    sysm.build_lists(("hydro", "grav"))
+    list_s += sysm.list_build_s
+    tune = None
+    if args.autotune:
+        sysm.gravity()                   # eta for the hydro trial runs
+        tune = sysm.autotune_groups((8, 16, 32), precision=MIXED)
+        log(f"autotuned walk group sizes {tune[0]} from {tune[1]}")
     log(f"lists built in {list_s:.2f}s")
     W = sysm.lists
     entries = {"density": list_entries(W["dens"]), "hydro": list_entries(W["hydro"]),
                "gravity": list_entries(W["grav"])}
     total_entries = sum(entries.values())
     flops = sum(entries[k] * FLOP_PER_ENTRY[k] for k in entries)
-
-    snap = {k: v.clone() for k, v in sysm.s.items()}
 
     def restore():
         for k, v in snap.items():
@@ -186,8 +206,10 @@ def main() -> int:
         "config": {"workload": "off-axis parabolic collision n=3 (M=1,R=1) + n=1.5 (M=0.5,R=0.6), "
                                "b=0.25R at 3R, one density+hydro+gravity force step",
                    "N": sysm.n, "eta": args.eta, "neighbours_target": 50, "theta": 0.5,
-                   "softening": 0.01, "n_group": args.n_group, "n_leaf": 8,
-                   "walks": W["dens"].n_walk},
+                   "softening": 0.01, "n_group": dict(sysm.n_group), "n_leaf": 8,
+                   "walks": {k: W[k].n_walk for k in W}},
+        "autotune_ms": None if tune is None else {k: {str(g): round(t, 4) for g, t in v.items()}
+                                                  for k, v in tune[1].items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "entries": entries, "flop_per_step": flops,

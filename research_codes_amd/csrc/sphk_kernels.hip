@@ -135,6 +135,36 @@ __device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v
     a.rz += dvx * dwy - dvy * dwx;
 }
 
+// Walks whose j-list holds at most kDensCap records keep the whole list in
+// LDS for the four sweeps (three h-passes, grad-h pass); longer lists stream
+// 64-record tiles.  kDensCap is sized so that the LDS footprint stays within
+// what the kernel's register-limited occupancy already allows.
+constexpr int kDensCap = 192;
+
+template <bool MIXED>
+__device__ __forceinline__ void dens_stage(int e, int t0_abs, int nj_end, const int32_t *__restrict__ j_idx,
+                                           const sphk_dens_src &pj, double4 *sp, R4<real_t<MIXED>> *sv,
+                                           int32_t *sid, bool with_v)
+{
+    using R = real_t<MIXED>;
+    const int k = t0_abs + e;
+    if (k < nj_end) {
+        const int j = j_idx ? j_idx[k] : k;
+        const double mj = pj.m[j];
+        sp[e] = make_double4(pj.x[j], pj.y[j], pj.z[j], mj);
+        if (with_v) {
+            sv[e] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)mj};
+            sid[e] = pj.id[j];
+        }
+    } else {
+        sp[e] = make_double4(kFar, kFar, kFar, 0.);
+        if (with_v) {
+            sv[e] = R4<R>{R(0), R(0), R(0), R(0)};
+            sid[e] = -1;
+        }
+    }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(kWave) void density_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
@@ -142,9 +172,9 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     double eta_ksrh, double hmax)
 {
     using R = real_t<MIXED>;
-    __shared__ double4 sp[kTile];     // x, y, z, m
-    __shared__ R4<R> sv[kTile];       // vx, vy, vz, m
-    __shared__ int32_t sid[kTile];
+    __shared__ double4 sp[kDensCap];     // x, y, z, m
+    __shared__ R4<R> sv[kDensCap];       // vx, vy, vz, m
+    __shared__ int32_t sid[kDensCap];
     __shared__ double red[5 * kWave];
     __shared__ double shs[kWave];
 
@@ -153,6 +183,9 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     const int lane = threadIdx.x;
     const int ib = i_off[w], ie = i_off[w + 1];
     const int jb = j_off[w], je = j_off[w + 1];
+    const int nj = je - jb;
+    const int ntile = (nj + kTile - 1) / kTile;
+    const bool resident = nj <= kDensCap;
 
     for (int c0 = ib; c0 < ie; c0 += kWave) {
         const Map M(min(kWave, ie - c0), lane);
@@ -164,30 +197,36 @@ __global__ __launch_bounds__(kWave) void density_kernel(
         double dens = 0.;
         long long np = 0;
 
+        if (resident) {
+            // one burst: every record of the list, padded to whole tiles
+            __syncthreads();
+            for (int e = lane; e < ntile * kTile; e += kWave)
+                dens_stage<MIXED>(e, jb, je, j_idx, pj, sp, sv, sid, true);
+            __syncthreads();
+        }
+
         // -- three fixed-point passes: rho(H_k), n(H_k), H_{k+1} (hdr_density.hpp:264-302)
         for (int rep = 0; rep < 3; rep++) {
             const double hi = 1. / hs;
             const double hi3 = hi * hi * hi;
             Pass1 a{0., 0};
-            for (int t0 = jb; t0 < je; t0 += kTile) {
-                const int nt = min(kTile, je - t0);
-                __syncthreads();
-                if (lane < nt) {
-                    const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                    sp[lane] = make_double4(pj.x[j], pj.y[j], pj.z[j], pj.m[j]);
-                } else {
-                    sp[lane] = make_double4(kFar, kFar, kFar, 0.);
+            for (int t = 0; t < ntile; t++) {
+                const double4 *tp = sp + t * kTile;
+                if (!resident) {
+                    __syncthreads();
+                    dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, sid, false);
+                    __syncthreads();
+                    tp = sp;
                 }
-                __syncthreads();
                 int k = M.slice;
                 for (; k + 3 * M.S < kTile; k += 4 * M.S) {
-                    const double4 j0 = sp[k], j1 = sp[k + M.S], j2 = sp[k + 2 * M.S], j3 = sp[k + 3 * M.S];
+                    const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
                     dens_pass1_pair(j0, px, py, pz, hi, hi3, a);
                     dens_pass1_pair(j1, px, py, pz, hi, hi3, a);
                     dens_pass1_pair(j2, px, py, pz, hi, hi3, a);
                     dens_pass1_pair(j3, px, py, pz, hi, hi3, a);
                 }
-                for (; k < kTile; k += M.S) dens_pass1_pair(sp[k], px, py, pz, hi, hi3, a);
+                for (; k < kTile; k += M.S) dens_pass1_pair(tp[k], px, py, pz, hi, hi3, a);
             }
             double v[2] = {a.rh, (double)a.nj};
             slice_reduce<2>(v, red, M, lane);
@@ -207,32 +246,27 @@ __global__ __launch_bounds__(kWave) void density_kernel(
         const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
         const R vx = (R)pi.vx[i], vy = (R)pi.vy[i], vz = (R)pi.vz[i];
         double acc[5] = {0., 0., 0., 0., 0.};
-        for (int t0 = jb; t0 < je; t0 += kTile) {
-            const int nt = min(kTile, je - t0);
-            __syncthreads();
-            if (lane < nt) {
-                const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                const double mj = pj.m[j];
-                sp[lane] = make_double4(pj.x[j], pj.y[j], pj.z[j], mj);
-                sv[lane] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)mj};
-                sid[lane] = pj.id[j];
-            } else {
-                sp[lane] = make_double4(kFar, kFar, kFar, 0.);
-                sv[lane] = R4<R>{R(0), R(0), R(0), R(0)};
-                sid[lane] = -1;
+        for (int t = 0; t < ntile; t++) {
+            const int base = resident ? t * kTile : 0;
+            if (!resident) {
+                __syncthreads();
+                dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, sid, true);
+                __syncthreads();
             }
-            __syncthreads();
+            const double4 *tp = sp + base;
+            const R4<R> *tv = sv + base;
+            const int32_t *td = sid + base;
             Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
             int k = M.slice;
             for (; k + M.S < kTile; k += 2 * M.S) {
-                const double4 p0 = sp[k], p1 = sp[k + M.S];
-                const R4<R> v0 = sv[k], v1 = sv[k + M.S];
-                const int32_t d0 = sid[k], d1 = sid[k + M.S];
+                const double4 p0 = tp[k], p1 = tp[k + M.S];
+                const R4<R> v0 = tv[k], v1 = tv[k + M.S];
+                const int32_t d0 = td[k], d1 = td[k + M.S];
                 dens_pass2_pair<R>(p0, v0, d0, px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
                 dens_pass2_pair<R>(p1, v1, d1, px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
             }
             for (; k < kTile; k += M.S)
-                dens_pass2_pair<R>(sp[k], sv[k], sid[k], px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
+                dens_pass2_pair<R>(tp[k], tv[k], td[k], px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
             acc[0] += (double)a.g;
             acc[1] += (double)a.dv;
             acc[2] += (double)a.rx;

@@ -50,7 +50,9 @@ class SPHSystem:
         self.eta = eta
         self.eps = eps
         self.theta = theta
-        self.n_group = n_group
+        # walk group size per list kind (an int applies to all three)
+        self.n_group = (dict(n_group) if isinstance(n_group, dict)
+                        else {"dens": n_group, "hydro": n_group, "grav": n_group})
         self.n_leaf = n_leaf
         self.precision = precision
         self.phys = phys_params or phys()
@@ -128,19 +130,63 @@ class SPHSystem:
         if "grav" in kinds:
             self._softening()
         eps2 = self.eps2.cpu().numpy()
-        tree = HostTree(x, y, z, m, eps2, self.n_leaf, self.n_group)
+        trees = {}
+
+        def tree_for(kind):
+            g = self.n_group[kind]
+            if g not in trees:
+                trees[g] = HostTree(x, y, z, m, eps2, self.n_leaf, g)
+            return trees[g]
+
         if "dens" in kinds:
             rs = self.rs.cpu().numpy()
-            self.lists["dens"] = Walks(tree.i_off, *tree.gather(rs), device=self.dev)
+            t = tree_for("dens")
+            self.lists["dens"] = Walks(t.i_off, *t.gather(rs), device=self.dev)
         if "hydro" in kinds:
-            self.lists["hydro"] = Walks(tree.i_off, *tree.symmetry(ksr), device=self.dev)
+            t = tree_for("hydro")
+            self.lists["hydro"] = Walks(t.i_off, *t.symmetry(ksr), device=self.dev)
         if "grav" in kinds:
-            j_off, j_idx, s_off, s_idx = tree.long(self.theta)
-            self.lists["grav"] = Walks(tree.i_off, j_off, j_idx, s_off, s_idx, device=self.dev)
-            self.mom = {k: to_device(v, self.dev) for k, v in tree.moments().items()}
-        tree.close()
+            t = tree_for("grav")
+            j_off, j_idx, s_off, s_idx = t.long(self.theta)
+            self.lists["grav"] = Walks(t.i_off, j_off, j_idx, s_off, s_idx, device=self.dev)
+            self.mom = {k: to_device(v, self.dev) for k, v in t.moments().items()}
+        for t in trees.values():
+            t.close()
         torch.cuda.synchronize()
         self.list_build_s = time.perf_counter() - t0
+
+    def autotune_groups(self, candidates=(8, 16, 32), reps: int = 3, precision=None):
+        """Pick, per list kind, the walk group size with the fastest kernel.
+
+        Each candidate rebuilds that kind's lists and times `reps` launches
+        of its functor (HIP events on the launch stream).  Leaves the best
+        lists in place; returns ({kind: size}, {kind: {size: ms}})."""
+        n = self.n
+        tmp = {k: torch.empty(n, dtype=torch.float64, device=self.dev)
+               for k in ("dens", "divv", "rotv", "grdh", "ksr")}
+        tmp["np"] = torch.empty(n, dtype=torch.int64, device=self.dev)
+        tmp["itr"] = torch.empty(n, dtype=torch.uint8, device=self.dev)
+        run = {"dens": lambda: self.density(out=tmp, precision=precision),
+               "hydro": lambda: self.hydro(precision),
+               "grav": lambda: self.gravity(precision)}
+        best, table = {}, {}
+        for kind in ("dens", "hydro", "grav"):
+            table[kind] = {}
+            for g in candidates:
+                self.n_group[kind] = g
+                self.build_lists((kind,))
+                run[kind]()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    run[kind]()
+                e1.record()
+                torch.cuda.synchronize()
+                table[kind][g] = e0.elapsed_time(e1) / reps
+            best[kind] = min(table[kind], key=table[kind].get)
+            self.n_group[kind] = best[kind]
+            self.build_lists((kind,))
+        return best, table
 
     # ---------------------------------------------------------------- kernels
     def _st(self):

@@ -143,3 +143,36 @@ def test_density_retry_loop_is_repeatable():
         for r in res[1:]:
             for a, b in zip(res[0], r):
                 assert torch.equal(a, b)
+
+
+def test_per_kernel_group_sizes_agree():
+    """Each functor may use its own walk grouping.  On one state with a search
+    radius well above H (so no intermediate h-pass is cut off by the list),
+    the FP64 force step must not depend on it beyond summation-order rounding
+    (theta = 0 makes gravity a direct sum, which does not depend on the groups)."""
+    p = ic.head_on(3000, eta=1.18)
+    with Context(0) as c:
+        s = SPHSystem(p, c, eta=1.18, eps=0.01, theta=0.0, precision=FP64, n_group=16)
+        s.initialize()
+        snap = {k: v.clone() for k, v in s.s.items()}
+        res = []
+        for groups in ((16, 16, 16), (8, 32, 64), (64, 8, 32)):
+            for k, v in snap.items():
+                s.s[k].copy_(v)
+            s.n_group.update(zip(("dens", "hydro", "grav"), groups))
+            s.rs.copy_(1.5 * s.s["ksr"])
+            s.build_lists(("dens",))
+            s.density()
+            assert int(s.itr.sum()) == 0          # no particle outgrew its search radius
+            s.eos_balsara()
+            s.build_lists(("hydro", "grav"))      # symmetry lists from the new H
+            s.gravity()
+            s.hydro()
+            res.append({k: s.s[k].clone() for k in ("dens", "ksr", "ax", "ay", "az", "udot", "pot")})
+    errs = {}
+    for b in res[1:]:
+        for k, a in res[0].items():
+            scale = a.abs().max().clamp_min(1e-300)
+            errs[k] = max(errs.get(k, 0.0), float(((a - b[k]).abs() / scale).max()))
+    print("[groups] max scaled diff", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) < 1e-12, errs
