@@ -44,6 +44,36 @@ __device__ __forceinline__ double rcp_f64(double x)
     return y;
 }
 
+// Cube root of a finite x > 0 in FP64.  x = f 2^(3k) with f in [0.5, 4); an
+// FP32 seed z ~ f^(-1/3) (v_log_f32, v_exp_f32) is refined by two
+// division-free Newton steps z <- z + z (1 - f z^3) / 3, then y = f z^2 gets
+// one Newton correction y <- y - (y^3 - f) z^2 / 3.  Within a few ulp of the
+// exact cube root.
+__device__ __forceinline__ double cbrt_pos_f64(double x)
+{
+    const int e = __builtin_amdgcn_frexp_exp(x);   // x = m 2^e, m in [0.5, 1)
+    const double m = __builtin_amdgcn_frexp_mant(x);
+    const int k = e >= 0 ? e / 3 : -((2 - e) / 3);
+    const double f = __builtin_amdgcn_ldexp(m, e - 3 * k);
+    double z = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)f) * (-1.f / 3.f));
+    double t = f * (z * z * z);
+    z = __builtin_fma(z * (1. / 3.), 1. - t, z);
+    t = f * (z * z * z);
+    z = __builtin_fma(z * (1. / 3.), 1. - t, z);
+    const double z2 = z * z;
+    double y = f * z2;
+    const double r = __builtin_fma(y * y, y, -f);
+    y = __builtin_fma(-r * (1. / 3.), z2, y);
+    return __builtin_amdgcn_ldexp(y, k);
+}
+
+// x^(1/3) with pow's special values for x <= 0, inf and NaN (x = 0 -> 0).
+__device__ __forceinline__ double cbrt_f64(double x)
+{
+    if (x > 0. && x < __builtin_inf()) return cbrt_pos_f64(x);
+    return x == 0. ? 0. : x;
+}
+
 __device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_r(double x) { return rsqrt_f64(x); }
 __device__ __forceinline__ float rcp_r(float x) { return __builtin_amdgcn_rcpf(x); }

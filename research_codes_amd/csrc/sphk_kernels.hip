@@ -109,10 +109,17 @@ struct Pass2 {
     R g, dv, rx, ry, rz;
 };
 
+// The id of a j-record travels in the fourth slot of its velocity record:
+// bit pattern in FP32, exact value in FP64.
+__device__ __forceinline__ float pack_id(int32_t id, float) { return __int_as_float(id); }
+__device__ __forceinline__ double pack_id(int32_t id, double) { return (double)id; }
+__device__ __forceinline__ int32_t unpack_id(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int32_t unpack_id(double v) { return (int32_t)v; }
+
 template <typename R>
-__device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v, int32_t jid,
-                                                double px, double py, double pz, R vx, R vy, R vz,
-                                                int32_t id_i, R hi, R hi4, Pass2<R> &a)
+__device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v, double px, double py,
+                                                double pz, R vx, R vy, R vz, int32_t id_i, R hi, R hi4,
+                                                Pass2<R> &a)
 {
     const R dpx = (R)(px - j.x);
     const R dpy = (R)(py - j.y);
@@ -121,11 +128,11 @@ __device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v
     const R dvy = vy - v.b;
     const R dvz = vz - v.c;
     const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-    const R ri = (id_i != jid) ? rsqrt_r(r2) : R(0);
+    const R ri = (id_i != unpack_id(v.d)) ? rsqrt_r(r2) : R(0);
     const R q = r2 * ri * hi;
     const R kw0 = kernel0th(q);
     const R kw1 = kernel1st(q);
-    const R mh = v.d * hi4;
+    const R mh = (R)j.w * hi4;
     a.g -= (R(3) * kw0 + q * kw1) * mh;
     const R dw = mh * kw1 * ri;
     const R dwx = dw * dpx, dwy = dw * dpy, dwz = dw * dpz;
@@ -135,34 +142,192 @@ __device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v
     a.rz += dvx * dwy - dvy * dwx;
 }
 
+// hdr_density.hpp:351-362: Omega^-1, div v and |rot v| from the grad-h sums.
+__device__ __forceinline__ void dens_finish(const sphk_dens_out &out, int i, const double (&acc)[5],
+                                            double dens, long long np, double hs, double rs)
+{
+    const double grdh = acc[0];
+    double divv = acc[1];
+    const double rot2 = acc[2] * acc[2] + acc[3] * acc[3] + acc[4] * acc[4];
+    const double deni = 1. / dens;
+    const double omgi = 1. / (1. + hs * deni * grdh / 3.);
+    double rotv = rot2 != 0. ? __builtin_sqrt(rot2) : 0.;
+    rotv *= deni * omgi;
+    divv *= deni * omgi;
+    out.dens[i] = dens;
+    out.np[i] = np;
+    out.ksr[i] = hs;
+    out.itr[i] = (hs > rs) ? 1 : 0;
+    out.divv[i] = divv;
+    out.rotv[i] = rotv;
+    out.grdh[i] = omgi;
+}
+
+// Wave-wide inclusive prefix sum and maximum.
+__device__ __forceinline__ int wave_scan_incl(int v, int lane)
+{
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_xor(v, d, kWave);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// New H from rho: hdr_density.hpp:296-300 (eta * kappa * (m / rho)^(1/3), capped).
+__device__ __forceinline__ double dens_next_h(double eta_ksrh, double m, double dens, double hmax)
+{
+    const double h = eta_ksrh * cbrt_f64(m / dens);
+    return h < hmax ? h : hmax;
+}
+
+// Per-i sums of the five grad-h partials over the slices of a full sweep.
+// The mixed kernel's reduction buffer holds two FP64 quantities, so the five
+// are combined two, two and one at a time.
+template <bool MIXED>
+__device__ __forceinline__ void dens_reduce_grad(double (&acc)[5], double *red, const Map &M, int lane)
+{
+    if (!MIXED) {
+        slice_reduce<5>(acc, red, M, lane);
+        return;
+    }
+    double a[2] = {acc[0], acc[1]};
+    slice_reduce<2>(a, red, M, lane);
+    double b[2] = {acc[2], acc[3]};
+    slice_reduce<2>(b, red, M, lane);
+    double c[1] = {acc[4]};
+    slice_reduce<1>(c, red, M, lane);
+    acc[0] = a[0];
+    acc[1] = a[1];
+    acc[2] = b[0];
+    acc[3] = b[1];
+    acc[4] = c[0];
+}
+
 // Walks whose j-list holds at most kDensCap records keep the whole list in
 // LDS for the four sweeps (three h-passes, grad-h pass); longer lists stream
-// 64-record tiles.  kDensCap is sized so that the LDS footprint stays within
-// what the kernel's register-limited occupancy already allows.
-constexpr int kDensCap = 192;
+// 64-record tiles.
+//
+// Resident lists are compacted per i-particle.  One FP64 sweep marks the
+// records within R_i = max(rs_i, H_0,i) (times 1 + 1e-12) of i; every record
+// outside contributes exactly zero to all four sweeps as long as H_k <= R_i,
+// so the sweeps visit only the marked records (positions in LDS, one byte
+// each).  The i-particles whose lists fit in kDensComp entries are taken
+// together; the lanes are shared out so that every taken i needs at most
+// T = ceil(C / (64 - n)) steps (C entries over n particles).  If an H_k of
+// the iteration exceeds R_i the chunk continues with full sweeps.
+constexpr int kDensCap = 256;
+constexpr int kDensComp = 1472;
+constexpr int kDensPad = kDensCap;   // inert record after the list, target of idle steps
+static_assert(kDensCap <= 256, "compacted positions are stored as bytes");
 
 template <bool MIXED>
 __device__ __forceinline__ void dens_stage(int e, int t0_abs, int nj_end, const int32_t *__restrict__ j_idx,
                                            const sphk_dens_src &pj, double4 *sp, R4<real_t<MIXED>> *sv,
-                                           int32_t *sid, bool with_v)
+                                           bool with_v)
 {
     using R = real_t<MIXED>;
     const int k = t0_abs + e;
     if (k < nj_end) {
         const int j = j_idx ? j_idx[k] : k;
-        const double mj = pj.m[j];
-        sp[e] = make_double4(pj.x[j], pj.y[j], pj.z[j], mj);
-        if (with_v) {
-            sv[e] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)mj};
-            sid[e] = pj.id[j];
-        }
+        sp[e] = make_double4(pj.x[j], pj.y[j], pj.z[j], pj.m[j]);
+        if (with_v) sv[e] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], pack_id(pj.id[j], R(0))};
     } else {
         sp[e] = make_double4(kFar, kFar, kFar, 0.);
-        if (with_v) {
-            sv[e] = R4<R>{R(0), R(0), R(0), R(0)};
-            sid[e] = -1;
-        }
+        if (with_v) sv[e] = R4<R>{R(0), R(0), R(0), pack_id(-1, R(0))};
     }
+}
+
+// The i-particle data one lane works with.
+struct DensI {
+    double x, y, z, m, ksr, rs, vx, vy, vz;
+    int32_t id;
+};
+
+__device__ __forceinline__ DensI dens_load_i(const sphk_dens_src &pi, int i)
+{
+    return DensI{pi.x[i], pi.y[i], pi.z[i], pi.m[i], pi.ksr[i], pi.rs[i], pi.vx[i], pi.vy[i], pi.vz[i],
+                 pi.id[i]};
+}
+
+// Lists longer than kDensCap: every sweep streams the list through one
+// 64-record LDS tile.
+template <bool MIXED>
+__device__ void density_streaming(int lane, const Map &M, int i, int jb, int je,
+                                  const int32_t *__restrict__ j_idx, const sphk_dens_src &pi,
+                                  const sphk_dens_src &pj, const sphk_dens_out &out, double eta_ksrh,
+                                  double hmax, double4 *sp, R4<real_t<MIXED>> *sv, double *red,
+                                  double *shs)
+{
+    using R = real_t<MIXED>;
+    const int ntile = (je - jb + kTile - 1) / kTile;
+    const DensI I = dens_load_i(pi, i);
+    double hs = I.ksr;
+    double dens = 0.;
+    long long np = 0;
+
+    for (int rep = 0; rep < 3; rep++) {
+        const double hi = 1. / hs;
+        const double hi3 = hi * hi * hi;
+        Pass1 a{0., 0};
+        for (int t = 0; t < ntile; t++) {
+            __syncthreads();
+            dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, false);
+            __syncthreads();
+            int k = M.slice;
+            for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+                const double4 j0 = sp[k], j1 = sp[k + M.S], j2 = sp[k + 2 * M.S], j3 = sp[k + 3 * M.S];
+                dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
+                dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
+                dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
+                dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
+            }
+            for (; k < kTile; k += M.S) dens_pass1_pair(sp[k], I.x, I.y, I.z, hi, hi3, a);
+        }
+        double v[2] = {a.rh, (double)a.nj};
+        slice_reduce<2>(v, red, M, lane);
+        if (M.lead) {
+            dens = v[0];
+            np = (long long)v[1];
+            shs[M.il] = dens_next_h(eta_ksrh, I.m, dens, hmax);
+        }
+        __syncthreads();
+        hs = shs[M.il];
+    }
+
+    const double hi_d = 1. / hs;
+    const R hi = (R)hi_d;
+    const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
+    const R vx = (R)I.vx, vy = (R)I.vy, vz = (R)I.vz;
+    double acc[5] = {0., 0., 0., 0., 0.};
+    for (int t = 0; t < ntile; t++) {
+        __syncthreads();
+        dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, true);
+        __syncthreads();
+        Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+        int k = M.slice;
+        for (; k + M.S < kTile; k += 2 * M.S) {
+            dens_pass2_pair<R>(sp[k], sv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+            dens_pass2_pair<R>(sp[k + M.S], sv[k + M.S], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+        }
+        for (; k < kTile; k += M.S)
+            dens_pass2_pair<R>(sp[k], sv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+        acc[0] += (double)a.g;
+        acc[1] += (double)a.dv;
+        acc[2] += (double)a.rx;
+        acc[3] += (double)a.ry;
+        acc[4] += (double)a.rz;
+    }
+    dens_reduce_grad<MIXED>(acc, red, M, lane);
+    if (M.lead) dens_finish(out, i, acc, dens, np, hs, I.rs);
 }
 
 template <bool MIXED>
@@ -172,11 +337,17 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     double eta_ksrh, double hmax)
 {
     using R = real_t<MIXED>;
-    __shared__ double4 sp[kDensCap];     // x, y, z, m
-    __shared__ R4<R> sv[kDensCap];       // vx, vy, vz, m
-    __shared__ int32_t sid[kDensCap];
-    __shared__ double red[5 * kWave];
+    constexpr int kWords = kDensCap / kTile;
+    constexpr int kRed = (MIXED ? 5 * kWave * 4 : 5 * kWave * 8) / 8;   // doubles: 5 partials per lane
+    __shared__ double4 sp[kDensCap + 1];     // x, y, z, m
+    __shared__ R4<R> sv[kDensCap + 1];       // vx, vy, vz, id
+    __shared__ uint8_t cidx[kDensComp];
+    __shared__ double red[kRed];
     __shared__ double shs[kWave];
+    __shared__ int s_own[kWave];
+    __shared__ uint16_t s_ci[kWave], s_coff[kWave];
+    int *s_cnt = reinterpret_cast<int *>(red);   // membership counts, before any reduction
+    R *redr = reinterpret_cast<R *>(red);        // grad-h partials of the compacted sweeps
 
     const int w = blockIdx.x;
     if (w >= n_walk) return;
@@ -185,114 +356,247 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     const int jb = j_off[w], je = j_off[w + 1];
     const int nj = je - jb;
     const int ntile = (nj + kTile - 1) / kTile;
-    const bool resident = nj <= kDensCap;
 
-    for (int c0 = ib; c0 < ie; c0 += kWave) {
+    if (nj > kDensCap) {
+        for (int c0 = ib; c0 < ie; c0 += kWave) {
+            const Map M(min(kWave, ie - c0), lane);
+            density_streaming<MIXED>(lane, M, c0 + M.il, jb, je, j_idx, pi, pj, out, eta_ksrh, hmax,
+                                     sp, sv, red, shs);
+        }
+        return;
+    }
+
+    // one burst: every record of the list, padded to whole tiles, plus the inert record
+    for (int e = lane; e < ntile * kTile; e += kWave) dens_stage<MIXED>(e, jb, je, j_idx, pj, sp, sv, true);
+    if (lane == 0) dens_stage<MIXED>(kDensPad, 0, 0, j_idx, pj, sp, sv, true);
+
+    for (int c0 = ib; c0 < ie;) {
         const Map M(min(kWave, ie - c0), lane);
-        const int i = c0 + M.il;
-        const int32_t id_i = pi.id[i];
-        const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
-        const double m_i = pi.m[i];
-        double hs = pi.ksr[i];
+        __syncthreads();   // staging done; LDS of the previous chunk no longer read
+
+        // -- membership sweep: lane (i, slice) marks its records within R_i
+        uint64_t wd[kWords];
+        int cnt = 0;
+        {
+            const int i = c0 + M.il;
+            const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
+            const double h0 = pi.ksr[i], rs = pi.rs[i];
+            const double rme = (h0 > rs ? h0 : rs) * (1. + 1e-12);
+            const double rm2 = rme * rme;
+#pragma unroll
+            for (int t = 0; t < kWords; t++) {
+                uint64_t b = 0;
+                if (t < ntile) {
+                    const double4 *tp = sp + t * kTile;
+                    int k = M.slice;
+                    for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+                        const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
+                        const double x0 = px - j0.x, y0 = py - j0.y, z0 = pz - j0.z;
+                        const double x1 = px - j1.x, y1 = py - j1.y, z1 = pz - j1.z;
+                        const double x2 = px - j2.x, y2 = py - j2.y, z2 = pz - j2.z;
+                        const double x3 = px - j3.x, y3 = py - j3.y, z3 = pz - j3.z;
+                        b |= (uint64_t)(x0 * x0 + y0 * y0 + z0 * z0 < rm2) << k;
+                        b |= (uint64_t)(x1 * x1 + y1 * y1 + z1 * z1 < rm2) << (k + M.S);
+                        b |= (uint64_t)(x2 * x2 + y2 * y2 + z2 * z2 < rm2) << (k + 2 * M.S);
+                        b |= (uint64_t)(x3 * x3 + y3 * y3 + z3 * z3 < rm2) << (k + 3 * M.S);
+                    }
+                    for (; k < kTile; k += M.S) {
+                        const double dx = px - tp[k].x, dy = py - tp[k].y, dz = pz - tp[k].z;
+                        b |= (uint64_t)(dx * dx + dy * dy + dz * dz < rm2) << k;
+                    }
+                }
+                wd[t] = b;
+                cnt += __popcll(b);
+            }
+            s_cnt[lane] = cnt;
+        }
+        __syncthreads();
+        int start = 0, ci = 0;
+        if (M.slice < M.S)
+            for (int s2 = 0; s2 < M.slice; s2++) start += s_cnt[M.il + M.ni * s2];
+        if (M.lead)
+            for (int s2 = 0; s2 < M.S; s2++) ci += s_cnt[M.il + M.ni * s2];
+        // take the leading i-particles whose lists fit in kDensComp entries
+        // (every single list fits: c_i <= kDensCap)
+        const int incl = wave_scan_incl(ci, lane);   // leads are lanes 0 .. ni-1
+        const bool fits = M.lead && incl <= kDensComp;
+        const int n_take = __popcll(__ballot(fits));
+        const int total = __shfl(incl, n_take - 1, kWave);
+        const bool take = M.il < n_take;             // lane works on a taken i
+
+        if (M.lead && take) {
+            s_coff[M.il] = (uint16_t)(incl - ci);
+            s_ci[M.il] = (uint16_t)ci;
+        }
+        s_own[lane] = -1;
+        __syncthreads();
+        if (take && M.slice < M.S) {
+            int o = s_coff[M.il] + start;
+#pragma unroll
+            for (int t = 0; t < kWords; t++) {
+                uint64_t b = wd[t];
+                while (b) {
+                    cidx[o++] = (uint8_t)(t * kTile + __builtin_ctzll(b));
+                    b &= b - 1;
+                }
+            }
+        }
+        // lanes per taken i: max(1, ceil(c_i / T)), T = ceil(C / (64 - n)); they sum to <= 64
+        const int tsteps = n_take < kWave ? max(1, (total + (kWave - n_take) - 1) / (kWave - n_take)) : 1;
+        const int si = (M.lead && take) ? (n_take < kWave ? max(1, (ci + tsteps - 1) / tsteps) : 1) : 0;
+        const int lincl = wave_scan_incl(si, lane);
+        if (si > 0)
+            for (int s2 = 0; s2 < si; s2++) s_own[lincl - si + s2] = M.il | (s2 << 8) | (si << 16);
+        __syncthreads();
+        const int own = s_own[lane];
+
+        // lane's particle and compacted range; every lane runs the wave's
+        // largest step count, idle steps read the inert record
+        int my = own >= 0 ? (own & 0xff) : 0;
+        int e0 = 0, n_e = 0, stride = 1;
+        const uint8_t *myidx = cidx;
+        if (own >= 0) {
+            e0 = (own >> 8) & 0xff;
+            stride = own >> 16;
+            n_e = s_ci[my];
+            myidx = cidx + s_coff[my];
+        }
+        const int nsteps = wave_max(n_e > e0 ? (n_e - e0 + stride - 1) / stride : 0);
+        bool compact = true;
+        bool owner = own >= 0 && e0 == 0;
+        DensI I = dens_load_i(pi, c0 + my);
+        const Map MF(n_take, lane);   // full sweeps over the taken i-particles
+        double hs = I.ksr;
         double dens = 0.;
         long long np = 0;
-
-        if (resident) {
-            // one burst: every record of the list, padded to whole tiles
-            __syncthreads();
-            for (int e = lane; e < ntile * kTile; e += kWave)
-                dens_stage<MIXED>(e, jb, je, j_idx, pj, sp, sv, sid, true);
-            __syncthreads();
-        }
 
         // -- three fixed-point passes: rho(H_k), n(H_k), H_{k+1} (hdr_density.hpp:264-302)
         for (int rep = 0; rep < 3; rep++) {
             const double hi = 1. / hs;
             const double hi3 = hi * hi * hi;
             Pass1 a{0., 0};
-            for (int t = 0; t < ntile; t++) {
-                const double4 *tp = sp + t * kTile;
-                if (!resident) {
-                    __syncthreads();
-                    dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, sid, false);
-                    __syncthreads();
-                    tp = sp;
+            double v[2];
+            if (compact) {
+                int st = 0, e = e0;
+                for (; st + 4 <= nsteps; st += 4, e += 4 * stride) {
+                    const int k0 = e < n_e ? myidx[e] : kDensPad;
+                    const int k1 = e + stride < n_e ? myidx[e + stride] : kDensPad;
+                    const int k2 = e + 2 * stride < n_e ? myidx[e + 2 * stride] : kDensPad;
+                    const int k3 = e + 3 * stride < n_e ? myidx[e + 3 * stride] : kDensPad;
+                    const double4 j0 = sp[k0], j1 = sp[k1], j2 = sp[k2], j3 = sp[k3];
+                    dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
+                    dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
+                    dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
+                    dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
                 }
-                int k = M.slice;
-                for (; k + 3 * M.S < kTile; k += 4 * M.S) {
-                    const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
-                    dens_pass1_pair(j0, px, py, pz, hi, hi3, a);
-                    dens_pass1_pair(j1, px, py, pz, hi, hi3, a);
-                    dens_pass1_pair(j2, px, py, pz, hi, hi3, a);
-                    dens_pass1_pair(j3, px, py, pz, hi, hi3, a);
+                for (; st < nsteps; st++, e += stride)
+                    dens_pass1_pair(sp[e < n_e ? myidx[e] : kDensPad], I.x, I.y, I.z, hi, hi3, a);
+                __syncthreads();
+                red[lane] = a.rh;
+                red[kWave + lane] = (double)a.nj;
+                __syncthreads();
+                v[0] = red[lane];
+                v[1] = red[kWave + lane];
+                if (owner)
+                    for (int t = 1; t < stride; t++) {
+                        v[0] += red[lane + t];
+                        v[1] += red[kWave + lane + t];
+                    }
+            } else {
+                for (int t = 0; t < ntile; t++) {
+                    const double4 *tp = sp + t * kTile;
+                    int k = MF.slice;
+                    for (; k + 3 * MF.S < kTile; k += 4 * MF.S) {
+                        const double4 j0 = tp[k], j1 = tp[k + MF.S], j2 = tp[k + 2 * MF.S], j3 = tp[k + 3 * MF.S];
+                        dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
+                        dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
+                        dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
+                        dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
+                    }
+                    for (; k < kTile; k += MF.S) dens_pass1_pair(tp[k], I.x, I.y, I.z, hi, hi3, a);
                 }
-                for (; k < kTile; k += M.S) dens_pass1_pair(tp[k], px, py, pz, hi, hi3, a);
+                v[0] = a.rh;
+                v[1] = (double)a.nj;
+                slice_reduce<2>(v, red, MF, lane);
             }
-            double v[2] = {a.rh, (double)a.nj};
-            slice_reduce<2>(v, red, M, lane);
-            if (M.lead) {
+            int viol = 0;
+            if (owner) {
                 dens = v[0];
                 np = (long long)v[1];
-                double hnew = eta_ksrh * pow(m_i / dens, 1. / 3.);
-                shs[M.il] = hnew < hmax ? hnew : hmax;
+                const double hnew = dens_next_h(eta_ksrh, I.m, dens, hmax);
+                shs[my] = hnew;
+                viol = compact && hnew > (I.ksr > I.rs ? I.ksr : I.rs);
             }
-            __syncthreads();
-            hs = shs[M.il];
+            // an H beyond R_i leaves the compacted lists incomplete: full
+            // sweeps from here on, the lead lanes of MF take over ownership
+            if (__syncthreads_or(viol)) {
+                if (owner) {
+                    red[my] = dens;
+                    red[kWave + my] = (double)np;
+                }
+                __syncthreads();
+                compact = false;
+                my = MF.il;
+                owner = MF.lead;
+                I = dens_load_i(pi, c0 + my);
+                dens = red[my];
+                np = (long long)red[kWave + my];
+            }
+            hs = shs[my];
         }
 
         // -- grad-h, divergence, curl with H_3 (hdr_density.hpp:304-349)
         const double hi_d = 1. / hs;
         const R hi = (R)hi_d;
         const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
-        const R vx = (R)pi.vx[i], vy = (R)pi.vy[i], vz = (R)pi.vz[i];
+        const R vx = (R)I.vx, vy = (R)I.vy, vz = (R)I.vz;
         double acc[5] = {0., 0., 0., 0., 0.};
-        for (int t = 0; t < ntile; t++) {
-            const int base = resident ? t * kTile : 0;
-            if (!resident) {
-                __syncthreads();
-                dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, sid, true);
-                __syncthreads();
-            }
-            const double4 *tp = sp + base;
-            const R4<R> *tv = sv + base;
-            const int32_t *td = sid + base;
+        if (compact) {
             Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
-            int k = M.slice;
-            for (; k + M.S < kTile; k += 2 * M.S) {
-                const double4 p0 = tp[k], p1 = tp[k + M.S];
-                const R4<R> v0 = tv[k], v1 = tv[k + M.S];
-                const int32_t d0 = td[k], d1 = td[k + M.S];
-                dens_pass2_pair<R>(p0, v0, d0, px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
-                dens_pass2_pair<R>(p1, v1, d1, px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
+            int st = 0, e = e0;
+            for (; st + 2 <= nsteps; st += 2, e += 2 * stride) {
+                const int k0 = e < n_e ? myidx[e] : kDensPad;
+                const int k1 = e + stride < n_e ? myidx[e + stride] : kDensPad;
+                dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                dens_pass2_pair<R>(sp[k1], sv[k1], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
             }
-            for (; k < kTile; k += M.S)
-                dens_pass2_pair<R>(tp[k], tv[k], td[k], px, py, pz, vx, vy, vz, id_i, hi, hi4, a);
-            acc[0] += (double)a.g;
-            acc[1] += (double)a.dv;
-            acc[2] += (double)a.rx;
-            acc[3] += (double)a.ry;
-            acc[4] += (double)a.rz;
+            if (st < nsteps) {
+                const int k0 = e < n_e ? myidx[e] : kDensPad;
+                dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+            }
+            const R part[5] = {a.g, a.dv, a.rx, a.ry, a.rz};
+            __syncthreads();
+            for (int q = 0; q < 5; q++) redr[q * kWave + lane] = part[q];
+            __syncthreads();
+            if (owner)
+                for (int q = 0; q < 5; q++) {
+                    double s = (double)redr[q * kWave + lane];
+                    for (int t = 1; t < stride; t++) s += (double)redr[q * kWave + lane + t];
+                    acc[q] = s;
+                }
+        } else {
+            for (int t = 0; t < ntile; t++) {
+                const double4 *tp = sp + t * kTile;
+                const R4<R> *tv = sv + t * kTile;
+                Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+                int k = MF.slice;
+                for (; k + MF.S < kTile; k += 2 * MF.S) {
+                    dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                    dens_pass2_pair<R>(tp[k + MF.S], tv[k + MF.S], I.x, I.y, I.z, vx, vy, vz, I.id, hi,
+                                       hi4, a);
+                }
+                for (; k < kTile; k += MF.S)
+                    dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                acc[0] += (double)a.g;
+                acc[1] += (double)a.dv;
+                acc[2] += (double)a.rx;
+                acc[3] += (double)a.ry;
+                acc[4] += (double)a.rz;
+            }
+            dens_reduce_grad<MIXED>(acc, red, MF, lane);
         }
-        slice_reduce<5>(acc, red, M, lane);
-
-        // hdr_density.hpp:351-362
-        if (M.lead) {
-            const double grdh = acc[0];
-            double divv = acc[1];
-            const double rot2 = acc[2] * acc[2] + acc[3] * acc[3] + acc[4] * acc[4];
-            const double deni = 1. / dens;
-            const double omgi = 1. / (1. + hs * deni * grdh / 3.);
-            double rotv = rot2 != 0. ? __builtin_sqrt(rot2) : 0.;
-            rotv *= deni * omgi;
-            divv *= deni * omgi;
-            out.dens[i] = dens;
-            out.np[i] = np;
-            out.ksr[i] = hs;
-            out.itr[i] = (hs > pi.rs[i]) ? 1 : 0;
-            out.divv[i] = divv;
-            out.rotv[i] = rotv;
-            out.grdh[i] = omgi;
-        }
+        if (owner) dens_finish(out, c0 + my, acc, dens, np, hs, I.rs);
+        c0 += n_take;
     }
 }
 
