@@ -896,6 +896,62 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
     }
 }
 
+// Mixed precision with uniform softening: the j-records of a walk are staged
+// as FP32 offsets from a reference point of the walk (its first i-particle),
+// formed in FP64, so the pair term needs no FP64 arithmetic.  The offset of
+// a far record carries a relative rounding error of 2^-24 of its distance;
+// for a near pair the error is 2^-24 of its distance from the reference point.
+__device__ __forceinline__ void gravity_pair_rel(const float4 &p, float xi, float yi, float zi, float e2,
+                                                 GravAcc<float> &a)
+{
+    const float dx = xi - p.x;
+    const float dy = yi - p.y;
+    const float dz = zi - p.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float rei = rsqrt_r(r2 + e2);
+    const float mr = p.w * rei;
+    const float w = mr * rei * rei;
+    a.pt += mr;
+    a.ax += dx * w;
+    a.ay += dy * w;
+    a.az += dz * w;
+    a.et += w;
+}
+
+__device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb, int ke,
+                                                 const int32_t *__restrict__ idx, const sphk_grav_src &src,
+                                                 double cx, double cy, double cz, float xi, float yi,
+                                                 float zi, float e2, float4 *sp, double (&acc)[5])
+{
+    for (int t0 = kb; t0 < ke; t0 += kTile) {
+        const int nt = min(kTile, ke - t0);
+        __syncthreads();
+        if (lane < nt) {
+            const int j = idx ? idx[t0 + lane] : t0 + lane;
+            sp[lane] = make_float4((float)(src.x[j] - cx), (float)(src.y[j] - cy), (float)(src.z[j] - cz),
+                                   (float)src.m[j]);
+        } else {
+            sp[lane] = make_float4(1e15f, 1e15f, 1e15f, 0.f);
+        }
+        __syncthreads();
+        GravAcc<float> a{0.f, 0.f, 0.f, 0.f, 0.f};
+        int k = M.slice;
+        for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+            const float4 p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
+            gravity_pair_rel(p0, xi, yi, zi, e2, a);
+            gravity_pair_rel(p1, xi, yi, zi, e2, a);
+            gravity_pair_rel(p2, xi, yi, zi, e2, a);
+            gravity_pair_rel(p3, xi, yi, zi, e2, a);
+        }
+        for (; k < kTile; k += M.S) gravity_pair_rel(sp[k], xi, yi, zi, e2, a);
+        acc[0] += (double)a.ax;
+        acc[1] += (double)a.ay;
+        acc[2] += (double)a.az;
+        acc[3] += (double)a.pt;
+        acc[4] += (double)a.et;
+    }
+}
+
 template <bool MIXED, bool UNIFORM>
 __global__ __launch_bounds__(kWave) void gravity_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
@@ -918,11 +974,22 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
         const int i = c0 + M.il;
         const double px = pi.x[i], py = pi.y[i], pz = pi.z[i], e2 = pi.eps2[i];
         double acc[5] = {0., 0., 0., 0., 0.};
-        gravity_list<MIXED, UNIFORM>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
-                                     acc);
-        if (s_off)
-            gravity_list<MIXED, UNIFORM>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
-                                         sp, acc);
+        if (MIXED && UNIFORM) {
+            const double cx = pi.x[ib], cy = pi.y[ib], cz = pi.z[ib];
+            const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
+            float4 *sr = reinterpret_cast<float4 *>(smem);
+            gravity_list_rel(M, lane, j_off[w], j_off[w + 1], j_idx, pj, cx, cy, cz, xi, yi, zi, (float)e2,
+                             sr, acc);
+            if (s_off)
+                gravity_list_rel(M, lane, s_off[w], s_off[w + 1], s_idx, ps, cx, cy, cz, xi, yi, zi,
+                                 (float)e2, sr, acc);
+        } else {
+            gravity_list<MIXED, UNIFORM>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
+                                         acc);
+            if (s_off)
+                gravity_list<MIXED, UNIFORM>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
+                                             sp, acc);
+        }
         slice_reduce<5>(acc, red, M, lane);
         if (M.lead) {
             if (UNIFORM) {
