@@ -64,7 +64,7 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="per functor")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="use --n-group for all kernels instead of tuning 8/16/32 per kernel")
+                    help="use --n-group for all kernels instead of tuning 8/16/32/64 per kernel")
     ap.add_argument("--groups", default=None,
                     help="walk group sizes dens,hydro,grav (e.g. 16,16,32); implies --no-autotune")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -121,7 +121,7 @@ def main() -> int:
     tune = None
     if args.autotune:
         sysm.gravity()                   # eta for the hydro trial runs
-        tune = sysm.autotune_groups((8, 16, 32), precision=MIXED)
+        tune = sysm.autotune_groups((8, 16, 32, 64), precision=MIXED)
         log(f"autotuned walk group sizes {tune[0]} from {tune[1]}")
     log(f"lists built in {list_s:.2f}s")
     W = sysm.lists
@@ -190,8 +190,10 @@ def main() -> int:
 
     dom = max(kern_mixed, key=lambda k: kern_mixed[k])
     achieved = entries[dom] * FLOP_PER_ENTRY[dom] / (kern_mixed[dom] * 1e-3)
+    traffic, traffic_src = profiled_traffic(dom, sysm.n)
     roofline = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
+                "traffic_source": traffic_src,
                 "flop_per_entry": FLOP_PER_ENTRY[dom], "entries_per_launch": entries[dom]}
 
     cpu = None
@@ -222,6 +224,19 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def profiled_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary of the same workload (profiles/traffic_<kernel>_N<n>_mixed.txt,
+    written by research_codes_amd.profiling), or (None, None)."""
+    path = ROOT / "profiles" / f"traffic_{kernel}_N{n:.0e}_mixed.txt".replace("+0", "")
+    if not path.exists():
+        return None, None
+    kv = dict(line.strip().split("=", 1) for line in path.read_text().splitlines() if "=" in line)
+    if "traffic_bytes_per_launch" not in kv:
+        return None, None
+    return float(kv["traffic_bytes_per_launch"]), str(path.relative_to(ROOT))
 
 
 def cpu_baseline(sysm, entries, seconds, log):

@@ -1,4 +1,4 @@
-"""ctypes binding of libsphk.so (include/sphk.h, include/sphk_tree.h).
+"""ctypes binding of libsphk.so (include/sphk.h, sphk_step.h, sphk_tree.h, sphk_dtree.h).
 
 The library is built in-tree by research_codes_amd.build.  Loading fails
 loudly when it is missing: there is no CPU fallback for the product path.
@@ -114,6 +114,24 @@ SIGNATURES = {
     "sphk_tree_moments": (C.c_int, [_P, C.POINTER(_DP), C.POINTER(_DP), C.POINTER(_DP),
                                     C.POINTER(_DP), C.POINTER(_DP)]),
 }
+
+SIGNATURES.update({
+    # sphk_dtree.h (all array arguments are device pointers)
+    "sphk_morton_order_dev": (C.c_int, [C.c_int, _P, _P, _P, _P, _P]),
+    "sphk_dtree_build": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)]),
+    "sphk_dtree_free": (None, [_P]),
+    "sphk_dtree_n_walk": (C.c_int, [_P]),
+    "sphk_dtree_n_node": (C.c_int, [_P]),
+    "sphk_dtree_i_off": (_P, [_P]),
+    "sphk_dtree_list_gather": (C.c_int, [_P, _P]),
+    "sphk_dtree_list_symmetry": (C.c_int, [_P, _P]),
+    "sphk_dtree_list_long": (C.c_int, [_P, C.c_double]),
+    "sphk_dtree_list": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                                  C.POINTER(_P), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "sphk_dtree_moments": (C.c_int, [_P] + [C.POINTER(_P)] * 5),
+    "sphk_dtree_nodes": (C.c_int, [_P] + [C.POINTER(_P)] * 6),
+    "sphk_copy_dev": (C.c_int, [_P, _P, C.c_size_t, _P]),
+})
 
 _lib = None
 

@@ -1,0 +1,934 @@
+// sphk_dtree.hip -- device-resident octree, walk groups and interaction lists.
+//
+// The host builder (sphk_tree.cpp) and this one implement the same
+// policies; this one keeps everything in HBM:
+//   1. Morton keys over the bounding cube (block reduction, one key per
+//      thread); sphk_morton_order_dev sorts (key, index) with a radix sort.
+//   2. The octree is built breadth-first, one kernel pair per level: count
+//      the non-empty child octants of every frontier node (binary searches
+//      in the sorted keys), exclusive scan, write the children.  Children of
+//      one level are numbered in parent order and octant order, as the
+//      host's breadth-first loop numbers them.
+//   3. Node boxes and monopoles bottom-up, one kernel per level, each node
+//      summing its particles (leaf) or children in index order with FP
+//      contraction off, so the FP64 sums round as on the host.
+//   4. Groups: a node is a walk if it holds <= n_group particles (or is a
+//      leaf) and its parent holds more; walks are ordered by first particle.
+//   5. Lists: one thread per walk runs the host's depth-first traversal
+//      (children pushed in reverse, popped in order) twice: count, then
+//      fill at scanned offsets; the output is the host's, element for
+//      element.
+#include <hip/hip_runtime.h>
+
+#include <cstring>   // rocPRIM's iterator headers use memset without including it
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cstdint>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "sphk.h"
+#include "sphk_dtree.h"
+#include "sphk_internal.hpp"
+
+using sphk::set_error;
+
+namespace {
+
+constexpr int kMaxLevel = 21;
+constexpr int kStack = 7 * kMaxLevel + 8;   // depth-first stack bound: 7 per level + 1
+constexpr int kBlock = 256;
+
+#define SPHK_TRY(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return set_error(SPHK_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Device array with capacity; grow() keeps the contents.
+template <typename T>
+struct DArr {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n, bool keep, hipStream_t st)
+    {
+        if (n <= cap) return hipSuccess;
+        T *q = nullptr;
+        hipError_t e = hipMalloc(&q, n * sizeof(T));
+        if (e != hipSuccess) return e;
+        if (keep && p && cap) e = hipMemcpyAsync(q, p, cap * sizeof(T), hipMemcpyDeviceToDevice, st);
+        // queued work may still read the old block: wait before freeing it
+        if (e == hipSuccess && p) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            (void)hipFree(q);
+            return e;
+        }
+        if (p) (void)hipFree(p);
+        p = q;
+        cap = n;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct DLists {
+    DArr<int32_t> j_off, j_idx, s_off, s_idx;
+    int64_t n_j = 0, n_s = 0;
+    bool built = false;
+};
+
+__device__ __forceinline__ uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+// ---- bounding cube and keys ------------------------------------------------
+__global__ __launch_bounds__(kBlock) void bbox_partial_kernel(int n, const double *__restrict__ x,
+                                                              const double *__restrict__ y,
+                                                              const double *__restrict__ z,
+                                                              double *__restrict__ part)
+{
+    __shared__ double s[6][kBlock];
+    const double inf = __builtin_inf();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double p[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], p[d]);
+            hi[d] = fmax(hi[d], p[d]);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        s[d][threadIdx.x] = lo[d];
+        s[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int d = 0; d < 3; d++) {
+                s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 6; d++) part[6 * blockIdx.x + d] = s[d][0];
+}
+
+// frame = {origin x, y, z, side}: the bounding cube, side enlarged by 1e-9
+// (the host's frame_of).
+__global__ void bbox_final_kernel(int nb, const double *__restrict__ part, double *__restrict__ frame)
+{
+    const double inf = __builtin_inf();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (int b = 0; b < nb; b++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], part[6 * b + d]);
+            hi[d] = fmax(hi[d], part[6 * b + 3 + d]);
+        }
+    double side = 0.;
+    for (int d = 0; d < 3; d++) side = fmax(side, hi[d] - lo[d]);
+    if (!(side > 0.)) side = 1.;
+    side *= 1. + 1e-9;
+    frame[0] = lo[0];
+    frame[1] = lo[1];
+    frame[2] = lo[2];
+    frame[3] = side;
+}
+
+__device__ __forceinline__ uint64_t quant(double v, double o, double s)
+{
+    double t = (v - o) * s;
+    if (!(t >= 0.)) t = 0.;
+    const uint64_t u = (uint64_t)t;
+    return u > 0x1fffff ? (uint64_t)0x1fffff : u;
+}
+
+__global__ __launch_bounds__(kBlock) void keys_kernel(int n, const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ z,
+                                                      const double *__restrict__ frame,
+                                                      uint64_t *__restrict__ key, int32_t *__restrict__ idx)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double s = double(1 << 21) / frame[3];
+    key[i] = spread3(quant(x[i], frame[0], s)) << 2 | spread3(quant(y[i], frame[1], s)) << 1 |
+             spread3(quant(z[i], frame[2], s));
+    if (idx) idx[i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void unsorted_kernel(int n, const uint64_t *__restrict__ key,
+                                                          int *__restrict__ flag)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 1 && i < n && key[i] < key[i - 1]) atomicOr(flag, 1);
+}
+
+// Sets up the frame and keys of n particles (queued on st).
+int make_keys(int n, const double *x, const double *y, const double *z, hipStream_t st, DArr<double> &part,
+              DArr<double> &frame, DArr<uint64_t> &key, int32_t *idx)
+{
+    const int nb = std::min(1024, (n + kBlock - 1) / kBlock);
+    SPHK_TRY(part.reserve(6 * (size_t)nb, false, st));
+    SPHK_TRY(frame.reserve(4, false, st));
+    SPHK_TRY(key.reserve(n, false, st));
+    hipLaunchKernelGGL(bbox_partial_kernel, dim3(nb), dim3(kBlock), 0, st, n, x, y, z, part.p);
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(1), 0, st, nb, part.p, frame.p);
+    hipLaunchKernelGGL(keys_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, x, y, z,
+                       frame.p, key.p, idx);
+    SPHK_TRY(hipGetLastError());
+    return SPHK_OK;
+}
+
+// ---- octree levels ---------------------------------------------------------
+struct NodeArrays {
+    int32_t *begin, *end, *child, *nchild, *level, *parent;
+    double *len;
+};
+
+// First index in [b, e) whose octant digit exceeds o (keys sorted).
+__device__ __forceinline__ int32_t octant_end(const uint64_t *__restrict__ key, int32_t b, int32_t e, int shift,
+                                              int o)
+{
+    while (b < e) {
+        const int32_t mid = b + ((e - b) >> 1);
+        if ((int)((key[mid] >> shift) & 7) <= o)
+            b = mid + 1;
+        else
+            e = mid;
+    }
+    return b;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void split_kernel(int fa, int nf, const uint64_t *__restrict__ key,
+                                                       NodeArrays nd, int n_leaf, int64_t *__restrict__ cnt,
+                                                       const int64_t *__restrict__ off, int base)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nf) return;
+    const int a = fa + k;
+    const int32_t nb = nd.begin[a], ne = nd.end[a], lv = nd.level[a];
+    if (ne - nb <= n_leaf || lv >= kMaxLevel) {
+        if (!WRITE) cnt[k] = 0;
+        return;
+    }
+    const int shift = 3 * (20 - lv);
+    int32_t b = nb;
+    int nch = 0;
+    const int first = WRITE ? base + (int)off[k] : 0;
+    for (int o = 0; o < 8 && b < ne; o++) {
+        const int32_t e = octant_end(key, b, ne, shift, o);
+        if (e > b) {
+            if (WRITE) {
+                const int c = first + nch;
+                nd.begin[c] = b;
+                nd.end[c] = e;
+                nd.child[c] = -1;
+                nd.nchild[c] = 0;
+                nd.level[c] = lv + 1;
+                nd.parent[c] = a;
+                nd.len[c] = nd.len[a] * 0.5;
+            }
+            nch++;
+        }
+        b = e;
+    }
+    if (WRITE) {
+        nd.child[a] = first;
+        nd.nchild[a] = nch;
+    } else {
+        cnt[k] = nch;
+    }
+}
+
+__global__ void root_kernel(int n, NodeArrays nd, const double *__restrict__ frame)
+{
+    nd.begin[0] = 0;
+    nd.end[0] = n;
+    nd.child[0] = -1;
+    nd.nchild[0] = 0;
+    nd.level[0] = 0;
+    nd.parent[0] = -1;
+    nd.len[0] = frame[3];
+}
+
+// ---- boxes and monopoles, one level at a time (bottom-up) ------------------
+struct Moments {
+    double *lo, *hi;        // [3][cap] tight box
+    double *mm, *mx, *my, *mz, *me;
+    double *sx, *sy, *sz, *se;   // mass-weighted sums
+    size_t cap;
+};
+
+__global__ __launch_bounds__(kBlock) void moments_kernel(int a0, int na, NodeArrays nd, Moments mo,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ y,
+                                                         const double *__restrict__ z,
+                                                         const double *__restrict__ m,
+                                                         const double *__restrict__ eps2)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= na) return;
+    const int a = a0 + k;
+    const size_t C = mo.cap;
+    const double inf = __builtin_inf();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    double M = 0., X = 0., Y = 0., Z = 0., E = 0.;
+    if (nd.child[a] < 0) {
+        for (int i = nd.begin[a]; i < nd.end[a]; i++) {
+            const double p[3] = {x[i], y[i], z[i]};
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmin(lo[d], p[d]);
+                hi[d] = fmax(hi[d], p[d]);
+            }
+            const double e2 = eps2 ? eps2[i] : 0.;
+            M += m[i];
+            X += m[i] * x[i];
+            Y += m[i] * y[i];
+            Z += m[i] * z[i];
+            E += m[i] * e2;
+        }
+    } else {
+        for (int c = nd.child[a]; c < nd.child[a] + nd.nchild[a]; c++) {
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmin(lo[d], mo.lo[d * C + c]);
+                hi[d] = fmax(hi[d], mo.hi[d * C + c]);
+            }
+            M += mo.mm[c];
+            X += mo.sx[c];
+            Y += mo.sy[c];
+            Z += mo.sz[c];
+            E += mo.se[c];
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        mo.lo[d * C + a] = lo[d];
+        mo.hi[d * C + a] = hi[d];
+    }
+    mo.mm[a] = M;
+    mo.sx[a] = X;
+    mo.sy[a] = Y;
+    mo.sz[a] = Z;
+    mo.se[a] = E;
+    if (M > 0.) {
+        mo.mx[a] = X / M;
+        mo.my[a] = Y / M;
+        mo.mz[a] = Z / M;
+        mo.me[a] = E / M;
+    } else {
+        mo.mx[a] = 0.5 * (lo[0] + hi[0]);
+        mo.my[a] = 0.5 * (lo[1] + hi[1]);
+        mo.mz[a] = 0.5 * (lo[2] + hi[2]);
+        mo.me[a] = 0.;
+    }
+}
+
+// Box of x_j +- h_j per node (symmetry lists), bottom-up.
+__global__ __launch_bounds__(kBlock) void obox_kernel(int a0, int na, NodeArrays nd, double *__restrict__ olo,
+                                                      double *__restrict__ ohi, size_t C,
+                                                      const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ z,
+                                                      const double *__restrict__ h)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= na) return;
+    const int a = a0 + k;
+    const double inf = __builtin_inf();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    if (nd.child[a] < 0) {
+        for (int i = nd.begin[a]; i < nd.end[a]; i++) {
+            const double p[3] = {x[i], y[i], z[i]};
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmin(lo[d], p[d] - h[i]);
+                hi[d] = fmax(hi[d], p[d] + h[i]);
+            }
+        }
+    } else {
+        for (int c = nd.child[a]; c < nd.child[a] + nd.nchild[a]; c++)
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmin(lo[d], olo[d * C + c]);
+                hi[d] = fmax(hi[d], ohi[d * C + c]);
+            }
+    }
+    for (int d = 0; d < 3; d++) {
+        olo[d * C + a] = lo[d];
+        ohi[d * C + a] = hi[d];
+    }
+}
+
+// ---- groups ------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void fill_kernel(int n, int32_t *__restrict__ a, int32_t v)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void group_mark_kernel(int nn, NodeArrays nd, int n_group,
+                                                            int32_t *__restrict__ head)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nn) return;
+    const bool small = nd.end[a] - nd.begin[a] <= n_group || nd.child[a] < 0;
+    const int p = nd.parent[a];
+    const bool opened = p < 0 || nd.end[p] - nd.begin[p] > n_group;
+    if (small && opened) head[nd.begin[a]] = a;
+}
+
+__global__ __launch_bounds__(kBlock) void head_flag_kernel(int n, const int32_t *__restrict__ head,
+                                                           int64_t *__restrict__ flag)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) flag[i] = head[i] >= 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void group_write_kernel(int n, const int32_t *__restrict__ head,
+                                                             const int64_t *__restrict__ pos,
+                                                             int32_t *__restrict__ group,
+                                                             int32_t *__restrict__ i_off, int ng)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (head[i] >= 0) {
+        group[pos[i]] = head[i];
+        i_off[pos[i]] = i;
+    }
+    if (i == 0) i_off[ng] = n;
+}
+
+// ---- lists -------------------------------------------------------------------
+struct TreeView {
+    NodeArrays nd;
+    Moments mo;
+    const double *olo, *ohi;
+    const double *x, *y, *z;
+    const int32_t *group, *i_off;
+};
+
+__device__ __forceinline__ bool box_overlaps(const double (&alo)[3], const double (&ahi)[3],
+                                             const double *blo, const double *bhi, size_t C, int b)
+{
+    for (int d = 0; d < 3; d++)
+        if (bhi[d * C + b] < alo[d] || blo[d * C + b] > ahi[d]) return false;
+    return true;
+}
+
+// Depth-first walk of one group; MODE 0 gather, 1 symmetry, 2 long.  With
+// FILL the EPJ / SPJ entries are written from jo / so, else counted.
+template <int MODE, bool FILL>
+__global__ __launch_bounds__(64) void list_kernel(int ng, TreeView t, const double *__restrict__ r,
+                                                  double inv_t2, int64_t *__restrict__ cj,
+                                                  int64_t *__restrict__ cs, const int32_t *__restrict__ jo,
+                                                  const int32_t *__restrict__ so, int32_t *__restrict__ jout,
+                                                  int32_t *__restrict__ sout, int *__restrict__ err)
+{
+#pragma clang fp contract(off)
+    const int g = blockIdx.x * 64 + threadIdx.x;
+    if (g >= ng) return;
+    const size_t C = t.mo.cap;
+    const int ib = t.i_off[g], ie = t.i_off[g + 1];
+    const int gn = t.group[g];
+    const double inf = __builtin_inf();
+    // box of the i search spheres (gather: rs; symmetry: h); group tight box
+    double olo[3] = {inf, inf, inf}, ohi[3] = {-inf, -inf, -inf};
+    if (MODE != 2)
+        for (int i = ib; i < ie; i++) {
+            const double p[3] = {t.x[i], t.y[i], t.z[i]};
+            for (int d = 0; d < 3; d++) {
+                olo[d] = fmin(olo[d], p[d] - r[i]);
+                ohi[d] = fmax(ohi[d], p[d] + r[i]);
+            }
+        }
+    double glo[3], ghi[3];
+    for (int d = 0; d < 3; d++) {
+        glo[d] = t.mo.lo[d * C + gn];
+        ghi[d] = t.mo.hi[d * C + gn];
+    }
+    int64_t nj = 0, ns = 0;
+    const int64_t j0 = FILL ? (int64_t)jo[g] : 0;
+    const int64_t s0 = (FILL && MODE == 2) ? (int64_t)so[g] : 0;
+    int32_t st[kStack];
+    int sp = 0;
+    st[sp++] = 0;
+    while (sp > 0) {
+        const int a = st[--sp];
+        const int c0 = t.nd.child[a], nc = t.nd.nchild[a];
+        bool open = false;
+        if (MODE == 0) {
+            if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a)) continue;
+            if (c0 < 0) {
+                for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
+                    const double p[3] = {t.x[j], t.y[j], t.z[j]};
+                    bool in = true;
+                    for (int d = 0; d < 3; d++) in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
+                    if (!in) continue;
+                    for (int i = ib; i < ie; i++) {
+                        const double dx = p[0] - t.x[i], dy = p[1] - t.y[i], dz = p[2] - t.z[i];
+                        if (dx * dx + dy * dy + dz * dz < r[i] * r[i]) {
+                            if (FILL) jout[j0 + nj] = j;
+                            nj++;
+                            break;
+                        }
+                    }
+                }
+            } else {
+                open = true;
+            }
+        } else if (MODE == 1) {
+            if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a) && !box_overlaps(glo, ghi, t.olo, t.ohi, C, a))
+                continue;
+            if (c0 < 0) {
+                for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
+                    const double p[3] = {t.x[j], t.y[j], t.z[j]};
+                    const double hj = r[j];
+                    bool in = true, jov = true;
+                    for (int d = 0; d < 3; d++) {
+                        in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
+                        jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
+                    }
+                    if (!in && !jov) continue;
+                    for (int i = ib; i < ie; i++) {
+                        const double dx = p[0] - t.x[i], dy = p[1] - t.y[i], dz = p[2] - t.z[i];
+                        const double hm = r[i] > hj ? r[i] : hj;
+                        if (dx * dx + dy * dy + dz * dz < hm * hm) {
+                            if (FILL) jout[j0 + nj] = j;
+                            nj++;
+                            break;
+                        }
+                    }
+                }
+            } else {
+                open = true;
+            }
+        } else {
+            if (t.mo.mm[a] == 0.) continue;
+            const double c[3] = {t.mo.mx[a], t.mo.my[a], t.mo.mz[a]};
+            double s = 0.;
+            for (int d = 0; d < 3; d++) {
+                double e = 0.;
+                if (c[d] < glo[d])
+                    e = glo[d] - c[d];
+                else if (c[d] > ghi[d])
+                    e = c[d] - ghi[d];
+                s += e * e;
+            }
+            const double L = t.nd.len[a];
+            if (s > L * L * inv_t2) {
+                if (FILL) sout[s0 + ns] = a;
+                ns++;
+            } else if (c0 < 0) {
+                for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
+                    if (FILL) jout[j0 + nj] = j;
+                    nj++;
+                }
+            } else {
+                open = true;
+            }
+        }
+        if (open) {
+            if (sp + nc > kStack) {   // cannot happen for depth <= 21; stop cleanly
+                atomicOr(err, 1);
+                break;
+            }
+            for (int cc = c0 + nc - 1; cc >= c0; cc--) st[sp++] = cc;
+        }
+    }
+    if (!FILL) {
+        cj[g] = nj;
+        if (MODE == 2) cs[g] = ns;
+    }
+}
+
+__global__ void offsets_kernel(int ng, const int64_t *__restrict__ scan, const int64_t *__restrict__ cnt,
+                               int32_t *__restrict__ off, int64_t *__restrict__ total)
+{
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g < ng) off[g] = (int32_t)scan[g];
+    if (g == ng - 1) {
+        const int64_t t = scan[g] + cnt[g];
+        *total = t;
+        off[ng] = (int32_t)t;
+    }
+}
+
+}  // namespace
+
+struct sphk_dtree {
+    int n = 0, n_leaf = 8, n_group = 16;
+    hipStream_t st = nullptr;
+    DArr<double> x, y, z, part, frame, len;
+    DArr<uint64_t> key;
+    DArr<int32_t> begin, end, child, nchild, level, parent;
+    std::vector<int> level_start;   // node index of the first node of each level
+    int n_node = 0;
+    DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
+    DArr<int32_t> head, group, i_off;
+    int n_walk = 0;
+    DArr<int64_t> i64a, i64b, i64c, i64d;
+    DArr<int64_t> total;
+    DArr<int> err;
+    DArr<char> scratch;
+    DLists lists[3];
+
+    NodeArrays nodes() { return NodeArrays{begin.p, end.p, child.p, nchild.p, level.p, parent.p, len.p}; }
+    Moments moments()
+    {
+        return Moments{box_lo.p, box_hi.p, mm.p, mx.p, my.p, mz.p, me.p, sx.p, sy.p, sz.p, se.p, box_lo.cap / 3};
+    }
+    void release()
+    {
+        for (auto *a : {&x, &y, &z, &part, &frame, &len, &box_lo, &box_hi, &mm, &mx, &my, &mz, &me, &sx, &sy,
+                        &sz, &se, &olo, &ohi})
+            a->release();
+        key.release();
+        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off}) a->release();
+        for (auto *a : {&i64a, &i64b, &i64c, &i64d, &total}) a->release();
+        err.release();
+        scratch.release();
+        for (auto &L : lists) {
+            L.j_off.release();
+            L.j_idx.release();
+            L.s_off.release();
+            L.s_idx.release();
+        }
+    }
+};
+
+namespace {
+
+int reserve_nodes(sphk_dtree *t, size_t cap)
+{
+    if (cap <= t->begin.cap) return SPHK_OK;
+    for (auto *a : {&t->begin, &t->end, &t->child, &t->nchild, &t->level, &t->parent})
+        SPHK_TRY(a->reserve(cap, true, t->st));
+    SPHK_TRY(t->len.reserve(cap, true, t->st));
+    return SPHK_OK;
+}
+
+// exclusive scan of n int64 counts (queued on st)
+int scan64(sphk_dtree *t, const int64_t *in, int64_t *out, int n)
+{
+    size_t bytes = 0;
+    SPHK_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(),
+                                     t->st));
+    SPHK_TRY(t->scratch.reserve(bytes + 16, false, t->st));
+    SPHK_TRY(rocprim::exclusive_scan(t->scratch.p, bytes, in, out, (int64_t)0, (size_t)n,
+                                     rocprim::plus<int64_t>(), t->st));
+    return SPHK_OK;
+}
+
+template <typename T>
+int read1(hipStream_t st, const T *dev, T *host)
+{
+    SPHK_TRY(hipMemcpyAsync(host, dev, sizeof(T), hipMemcpyDeviceToHost, st));
+    SPHK_TRY(hipStreamSynchronize(st));
+    return SPHK_OK;
+}
+
+int grid(int n) { return (n + kBlock - 1) / kBlock; }
+
+template <int MODE>
+int build_list(sphk_dtree *t, const double *r, double inv_t2)
+{
+    DLists &L = t->lists[MODE];
+    L.built = false;
+    const int ng = t->n_walk;
+    SPHK_TRY(t->i64a.reserve(ng, false, t->st));   // EPJ counts
+    SPHK_TRY(t->i64b.reserve(ng, false, t->st));   // SPJ counts
+    SPHK_TRY(t->i64c.reserve(ng, false, t->st));   // scans
+    SPHK_TRY(t->total.reserve(2, false, t->st));
+    SPHK_TRY(t->err.reserve(1, false, t->st));
+    SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
+    if (MODE == 2) SPHK_TRY(L.s_off.reserve(ng + 1, false, t->st));
+    SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
+    TreeView v{t->nodes(), t->moments(), t->olo.p, t->ohi.p, t->x.p, t->y.p, t->z.p, t->group.p, t->i_off.p};
+    const dim3 lg((ng + 63) / 64), lb(64);
+    hipLaunchKernelGGL((list_kernel<MODE, false>), lg, lb, 0, t->st, ng, v, r, inv_t2, t->i64a.p, t->i64b.p,
+                       nullptr, nullptr, nullptr, nullptr, t->err.p);
+    SPHK_TRY(hipGetLastError());
+    int64_t tot[2] = {0, 0};
+    if (int rc = scan64(t, t->i64a.p, t->i64c.p, ng)) return rc;
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i64c.p, t->i64a.p,
+                       L.j_off.p, t->total.p);
+    if (int rc = read1(t->st, t->total.p, &tot[0])) return rc;
+    if (MODE == 2) {
+        if (int rc = scan64(t, t->i64b.p, t->i64c.p, ng)) return rc;
+        hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i64c.p, t->i64b.p,
+                           L.s_off.p, t->total.p + 1);
+        if (int rc = read1(t->st, t->total.p + 1, &tot[1])) return rc;
+    }
+    if (tot[0] > std::numeric_limits<int32_t>::max() || tot[1] > std::numeric_limits<int32_t>::max())
+        return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
+    SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
+    if (MODE == 2) SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
+    hipLaunchKernelGGL((list_kernel<MODE, true>), lg, lb, 0, t->st, ng, v, r, inv_t2, nullptr, nullptr,
+                       L.j_off.p, MODE == 2 ? L.s_off.p : nullptr, L.j_idx.p,
+                       MODE == 2 ? L.s_idx.p : nullptr, t->err.p);
+    SPHK_TRY(hipGetLastError());
+    int e = 0;
+    if (int rc = read1(t->st, t->err.p, &e)) return rc;
+    if (e) return set_error(SPHK_E_STATE, "dtree: traversal stack exceeded");
+    L.n_j = tot[0];
+    L.n_s = MODE == 2 ? tot[1] : 0;
+    L.built = true;
+    return SPHK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphk_morton_order_dev(int n, const double *x, const double *y, const double *z, int32_t *perm, void *stream)
+{
+    if (n < 0 || (n > 0 && (!x || !y || !z || !perm))) return set_error(SPHK_E_ARG, "morton_dev: bad args");
+    if (n == 0) return SPHK_OK;
+    hipStream_t st = (hipStream_t)stream;
+    DArr<double> part, frame;
+    DArr<uint64_t> key, key2;
+    DArr<int32_t> idx;
+    DArr<char> tmp;
+    int rc = SPHK_OK;
+    do {
+        if (idx.reserve(n, false, st) != hipSuccess || key2.reserve(n, false, st) != hipSuccess) {
+            rc = set_error(SPHK_E_DEVICE, "morton_dev: allocation failed");
+            break;
+        }
+        if ((rc = make_keys(n, x, y, z, st, part, frame, key, idx.p))) break;
+        size_t bytes = 0;
+        hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, key.p, key2.p, idx.p, perm, (size_t)n, 0, 63, st);
+        if (e == hipSuccess) e = tmp.reserve(bytes + 16, false, st);
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(tmp.p, bytes, key.p, key2.p, idx.p, perm, (size_t)n, 0, 63, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = set_error(SPHK_E_DEVICE, "morton_dev: %s", hipGetErrorString(e));
+    } while (0);
+    part.release();
+    frame.release();
+    key.release();
+    key2.release();
+    idx.release();
+    tmp.release();
+    return rc;
+}
+
+int sphk_dtree_build(int n, const double *x, const double *y, const double *z, const double *m,
+                     const double *eps2, int n_leaf, int n_group, void *stream, sphk_dtree **out)
+{
+    if (!out) return set_error(SPHK_E_ARG, "dtree: null output");
+    *out = nullptr;
+    if (n <= 0 || !x || !y || !z || !m) return set_error(SPHK_E_ARG, "dtree: bad particle arrays");
+    if (n_leaf < 1 || n_group < 1) return set_error(SPHK_E_ARG, "dtree: n_leaf, n_group must be >= 1");
+    sphk_dtree *t = new (std::nothrow) sphk_dtree();
+    if (!t) return set_error(SPHK_E_CAPACITY, "dtree: out of memory");
+    t->n = n;
+    t->n_leaf = n_leaf;
+    t->n_group = n_group;
+    t->st = (hipStream_t)stream;
+    auto fail = [&](int rc) {
+        t->release();
+        delete t;
+        return rc;
+    };
+    hipStream_t st = t->st;
+    // own copies of the positions (list builds read them later)
+    for (auto pr : {std::make_pair(&t->x, x), std::make_pair(&t->y, y), std::make_pair(&t->z, z)}) {
+        if (pr.first->reserve(n, false, st) != hipSuccess ||
+            hipMemcpyAsync(pr.first->p, pr.second, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return fail(set_error(SPHK_E_DEVICE, "dtree: position copy failed"));
+    }
+    if (int rc = make_keys(n, t->x.p, t->y.p, t->z.p, st, t->part, t->frame, t->key, nullptr)) return fail(rc);
+    if (t->err.reserve(1, false, st) != hipSuccess || hipMemsetAsync(t->err.p, 0, sizeof(int), st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    hipLaunchKernelGGL(unsorted_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->key.p, t->err.p);
+    int unsorted = 0;
+    if (int rc = read1(st, t->err.p, &unsorted)) return fail(rc);
+    if (unsorted)
+        return fail(set_error(SPHK_E_ARG, "dtree: particles are not in Morton order (use sphk_morton_order_dev)"));
+
+    // root, then one level at a time
+    if (int rc = reserve_nodes(t, 2 * (size_t)n / std::max(1, n_leaf) + 16)) return fail(rc);
+    hipLaunchKernelGGL(root_kernel, dim3(1), dim3(1), 0, st, n, t->nodes(), t->frame.p);
+    t->level_start = {0, 1};
+    int fa = 0, fb = 1;
+    while (fa < fb) {
+        const int nf = fb - fa;
+        if (t->i64a.reserve(nf, false, st) != hipSuccess || t->i64c.reserve(nf, false, st) != hipSuccess ||
+            t->total.reserve(2, false, st) != hipSuccess)
+            return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+        hipLaunchKernelGGL((split_kernel<false>), dim3(grid(nf)), dim3(kBlock), 0, st, fa, nf, t->key.p,
+                           t->nodes(), n_leaf, t->i64a.p, nullptr, 0);
+        if (int rc = scan64(t, t->i64a.p, t->i64c.p, nf)) return fail(rc);
+        int64_t last[2];
+        if (hipMemcpyAsync(&last[0], t->i64c.p + nf - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(&last[1], t->i64a.p + nf - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return fail(set_error(SPHK_E_DEVICE, "dtree: level size read failed"));
+        const int64_t nnew = last[0] + last[1];
+        if (nnew == 0) break;
+        if (fb + nnew > std::numeric_limits<int32_t>::max())
+            return fail(set_error(SPHK_E_CAPACITY, "dtree: too many nodes"));
+        if ((size_t)(fb + nnew) > t->begin.cap)
+            if (int rc = reserve_nodes(t, std::max<size_t>((size_t)(fb + nnew), t->begin.cap * 3 / 2))) return fail(rc);
+        hipLaunchKernelGGL((split_kernel<true>), dim3(grid(nf)), dim3(kBlock), 0, st, fa, nf, t->key.p, t->nodes(),
+                           n_leaf, nullptr, t->i64c.p, fb);
+        fa = fb;
+        fb = fb + (int)nnew;
+        t->level_start.push_back(fb);
+    }
+    t->n_node = fb;
+    const size_t nn = (size_t)fb;
+
+    // boxes and monopoles, deepest level first
+    for (auto *a : {&t->mm, &t->mx, &t->my, &t->mz, &t->me, &t->sx, &t->sy, &t->sz, &t->se})
+        if (a->reserve(nn, false, st) != hipSuccess) return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    if (t->box_lo.reserve(3 * nn, false, st) != hipSuccess || t->box_hi.reserve(3 * nn, false, st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    // the box arrays are [3][cap]; cap = box_lo.cap / 3 must match box_hi's
+    if (t->box_lo.cap != t->box_hi.cap) return fail(set_error(SPHK_E_STATE, "dtree: box capacity mismatch"));
+    for (int L = (int)t->level_start.size() - 2; L >= 0; L--) {
+        const int a0 = t->level_start[L], na = t->level_start[L + 1] - a0;
+        hipLaunchKernelGGL(moments_kernel, dim3(grid(na)), dim3(kBlock), 0, st, a0, na, t->nodes(), t->moments(),
+                           t->x.p, t->y.p, t->z.p, m, eps2);
+    }
+    if (hipGetLastError() != hipSuccess) return fail(set_error(SPHK_E_DEVICE, "dtree: moments launch failed"));
+
+    // groups
+    if (t->head.reserve(n, false, st) != hipSuccess || t->i64a.reserve(n, false, st) != hipSuccess ||
+        t->i64c.reserve(n, false, st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    hipLaunchKernelGGL(fill_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, -1);
+    hipLaunchKernelGGL(group_mark_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, st, (int)nn, t->nodes(), n_group,
+                       t->head.p);
+    hipLaunchKernelGGL(head_flag_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64a.p);
+    if (int rc = scan64(t, t->i64a.p, t->i64c.p, n)) return fail(rc);
+    int64_t last[2];
+    if (hipMemcpyAsync(&last[0], t->i64c.p + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&last[1], t->i64a.p + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: group count read failed"));
+    const int ng = (int)(last[0] + last[1]);
+    if (t->group.reserve(ng, false, st) != hipSuccess || t->i_off.reserve(ng + 1, false, st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    hipLaunchKernelGGL(group_write_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64c.p, t->group.p,
+                       t->i_off.p, ng);
+    if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: build failed"));
+    t->n_walk = ng;
+    *out = t;
+    return SPHK_OK;
+}
+
+void sphk_dtree_free(sphk_dtree *t)
+{
+    if (!t) return;
+    (void)hipStreamSynchronize(t->st);
+    t->release();
+    delete t;
+}
+
+int sphk_dtree_n_walk(const sphk_dtree *t) { return t ? t->n_walk : -1; }
+int sphk_dtree_n_node(const sphk_dtree *t) { return t ? t->n_node : -1; }
+const int32_t *sphk_dtree_i_off(const sphk_dtree *t) { return t ? t->i_off.p : nullptr; }
+
+int sphk_dtree_list_gather(sphk_dtree *t, const double *rs)
+{
+    if (!t || !rs) return set_error(SPHK_E_ARG, "dtree gather: null argument");
+    return build_list<0>(t, rs, 0.);
+}
+
+int sphk_dtree_list_symmetry(sphk_dtree *t, const double *h)
+{
+    if (!t || !h) return set_error(SPHK_E_ARG, "dtree symmetry: null argument");
+    const size_t nn = (size_t)t->n_node;
+    if (t->olo.reserve(3 * nn, false, t->st) != hipSuccess || t->ohi.reserve(3 * nn, false, t->st) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "dtree symmetry: allocation failed");
+    // the obox arrays use the same [3][cap] stride as the tight boxes
+    const size_t C = t->box_lo.cap / 3;
+    if (t->olo.cap < 3 * C || t->ohi.cap < 3 * C) {
+        t->olo.release();
+        t->ohi.release();
+        if (t->olo.reserve(3 * C, false, t->st) != hipSuccess || t->ohi.reserve(3 * C, false, t->st) != hipSuccess)
+            return set_error(SPHK_E_DEVICE, "dtree symmetry: allocation failed");
+    }
+    for (int L = (int)t->level_start.size() - 2; L >= 0; L--) {
+        const int a0 = t->level_start[L], na = t->level_start[L + 1] - a0;
+        hipLaunchKernelGGL(obox_kernel, dim3(grid(na)), dim3(kBlock), 0, t->st, a0, na, t->nodes(), t->olo.p,
+                           t->ohi.p, C, t->x.p, t->y.p, t->z.p, h);
+    }
+    SPHK_TRY(hipGetLastError());
+    return build_list<1>(t, h, 0.);
+}
+
+int sphk_dtree_list_long(sphk_dtree *t, double theta)
+{
+    if (!t || !(theta >= 0.)) return set_error(SPHK_E_ARG, "dtree long: bad argument");
+    const double inv_t2 = theta > 0. ? 1. / (theta * theta) : std::numeric_limits<double>::infinity();
+    return build_list<2>(t, nullptr, inv_t2);
+}
+
+int sphk_dtree_list(const sphk_dtree *t, int kind, const int32_t **j_off, const int32_t **j_idx,
+                    const int32_t **s_off, const int32_t **s_idx, int64_t *n_j, int64_t *n_s)
+{
+    if (!t || kind < 0 || kind > 2) return set_error(SPHK_E_ARG, "dtree list: bad argument");
+    const DLists &L = t->lists[kind];
+    if (!L.built) return set_error(SPHK_E_STATE, "dtree list kind %d not built", kind);
+    if (j_off) *j_off = L.j_off.p;
+    if (j_idx) *j_idx = L.j_idx.p;
+    if (s_off) *s_off = kind == 2 ? L.s_off.p : nullptr;
+    if (s_idx) *s_idx = kind == 2 ? L.s_idx.p : nullptr;
+    if (n_j) *n_j = L.n_j;
+    if (n_s) *n_s = L.n_s;
+    return SPHK_OK;
+}
+
+int sphk_dtree_moments(const sphk_dtree *t, const double **m, const double **x, const double **y,
+                       const double **z, const double **eps2)
+{
+    if (!t) return set_error(SPHK_E_ARG, "dtree moments: null tree");
+    if (m) *m = t->mm.p;
+    if (x) *x = t->mx.p;
+    if (y) *y = t->my.p;
+    if (z) *z = t->mz.p;
+    if (eps2) *eps2 = t->me.p;
+    return SPHK_OK;
+}
+
+int sphk_dtree_nodes(const sphk_dtree *t, const int32_t **begin, const int32_t **end, const int32_t **child,
+                     const int32_t **nchild, const int32_t **level, const double **len)
+{
+    if (!t) return set_error(SPHK_E_ARG, "dtree nodes: null tree");
+    if (begin) *begin = t->begin.p;
+    if (end) *end = t->end.p;
+    if (child) *child = t->child.p;
+    if (nchild) *nchild = t->nchild.p;
+    if (level) *level = t->level.p;
+    if (len) *len = t->len.p;
+    return SPHK_OK;
+}
+
+int sphk_copy_dev(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return SPHK_OK;
+    if (!dst || !src) return set_error(SPHK_E_ARG, "copy_dev: null pointer");
+    SPHK_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return SPHK_OK;
+}
+
+}  // extern "C"

@@ -52,6 +52,9 @@ class Walks:
     """A walk table plus one list per walk, resident on the device."""
 
     def __init__(self, i_off, j_off, j_idx=None, s_off=None, s_idx=None, device="cuda"):
+        if isinstance(i_off, torch.Tensor):
+            self._from_tensors(i_off, j_off, j_idx, s_off, s_idx)
+            return
         self.n_walk = len(i_off) - 1
         self.i_off = to_device(np.asarray(i_off, np.int32), device)
         self.j_off = to_device(np.asarray(j_off, np.int32), device)
@@ -70,6 +73,21 @@ class Walks:
             self.entries_spj = int((ni * np.diff(np.asarray(s_off, np.int64))).sum())
         else:
             self.entries_spj = 0
+
+
+    def _from_tensors(self, i_off, j_off, j_idx, s_off, s_idx):
+        """Lists already on the device (int32 tensors, e.g. from DeviceTree)."""
+        for t in (i_off, j_off, j_idx, s_off, s_idx):
+            if t is not None and (t.dtype != torch.int32 or not t.is_cuda):
+                raise TypeError("device walk tables must be int32 CUDA tensors")
+        self.n_walk = i_off.numel() - 1
+        self.i_off, self.j_off, self.j_idx, self.s_off, self.s_idx = i_off, j_off, j_idx, s_off, s_idx
+        ni = (i_off[1:] - i_off[:-1]).long()
+        self.n_i = int(i_off[-1])
+        self.n_j = int(j_off[-1])
+        self.n_s = 0 if s_off is None else int(s_off[-1])
+        self.entries = int((ni * (j_off[1:] - j_off[:-1]).long()).sum())
+        self.entries_spj = 0 if s_off is None else int((ni * (s_off[1:] - s_off[:-1]).long()).sum())
 
 
 class Context:

@@ -1,0 +1,106 @@
+"""Summaries of rocprofv3 output for the interaction kernels.
+
+Reads the CSV files that `rocprofv3 --kernel-trace --stats --output-format csv`
+and `rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE --output-format csv` write, and
+reduces them to the figures the benchmark reports:
+
+* per-kernel launch counts and durations, with the mean over the last K
+  launches of each kernel (the timed steps of a bench.py run);
+* HBM bytes per launch of one kernel from FETCH_SIZE / WRITE_SIZE (KiB per
+  dispatch), each scaled by what the same counter reads for a plain
+  device-to-device copy of known size (bench.py --copy-calibration runs three
+  256 MiB copies first).
+
+Usage:
+  python -m research_codes_amd.profiling trace <kernel_stats.csv> <kernel_trace.csv> [last]
+  python -m research_codes_amd.profiling traffic <fetch counter_collection.csv> \
+      <write counter_collection.csv> <kernel substring> [last]
+"""
+from __future__ import annotations
+
+import csv
+import statistics
+import sys
+from collections import OrderedDict, defaultdict
+
+COPY_BYTES = 256 * 1024 * 1024     # one calibration copy reads and writes this much
+COPY_KERNEL = "copyBuffer"
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "")
+
+
+def kernel_trace(stats_csv: str, trace_csv: str, last: int = 5) -> list[dict]:
+    """Per kernel: calls, mean over all launches and over the last `last` (ms)."""
+    durs: dict[str, list[float]] = defaultdict(list)
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            durs[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+    rows = []
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            d = durs.get(r["Name"], [])
+            rows.append({"kernel": _short(r["Name"]), "calls": int(r["Calls"]),
+                         "avg_ms": float(r["AverageNs"]) * 1e-6,
+                         "last_avg_ms": statistics.fmean(d[-last:]) if d else float("nan"),
+                         "percent": float(r["Percentage"])})
+    return rows
+
+
+def _per_dispatch(path: str) -> "OrderedDict[int, tuple[str, float]]":
+    out: "OrderedDict[int, tuple[str, float]]" = OrderedDict()
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            k = int(r["Dispatch_Id"])
+            name, v = out.get(k, (r["Kernel_Name"], 0.0))
+            out[k] = (name, v + float(r["Counter_Value"]))
+    return out
+
+
+def traffic(fetch_csv: str, write_csv: str, kernel: str, last: int = 5) -> dict:
+    """HBM bytes per launch of `kernel` (substring of its name), calibrated.
+
+    The calibration copies are the copy dispatches whose counter reads above a
+    quarter of COPY_BYTES; the scale is COPY_BYTES over their mean reading."""
+    res = {}
+    for label, path in (("fetch", fetch_csv), ("write", write_csv)):
+        d = _per_dispatch(path)
+        copies = [v for name, v in d.values() if COPY_KERNEL in name and v * 1024 > COPY_BYTES / 4]
+        if not copies:
+            raise ValueError(f"{path}: no calibration copy dispatch found")
+        scale = COPY_BYTES / (statistics.fmean(copies) * 1024)
+        vals = [v for name, v in d.values() if kernel in name][-last:]
+        if not vals:
+            raise ValueError(f"{path}: no dispatch of {kernel}")
+        res[label] = {"kib_per_launch": statistics.fmean(vals), "scale": scale,
+                      "bytes_per_launch": statistics.fmean(vals) * 1024 * scale}
+    res["bytes_per_launch"] = res["fetch"]["bytes_per_launch"] + res["write"]["bytes_per_launch"]
+    return res
+
+
+def main(argv: list[str]) -> int:
+    if len(argv) >= 3 and argv[0] == "trace":
+        last = int(argv[3]) if len(argv) > 3 else 5
+        print(f"| kernel | calls | avg all (ms) | avg last {last} (ms) | % time |")
+        print("|---|---|---|---|---|")
+        for r in kernel_trace(argv[1], argv[2], last):
+            print(f"| `{r['kernel']}` | {r['calls']} | {r['avg_ms']:.4f} | {r['last_avg_ms']:.4f} | "
+                  f"{r['percent']:.2f} |")
+        return 0
+    if len(argv) >= 4 and argv[0] == "traffic":
+        last = int(argv[4]) if len(argv) > 4 else 5
+        t = traffic(argv[1], argv[2], argv[3], last)
+        print(f"kernel={argv[3]}")
+        for k in ("fetch", "write"):
+            print(f"{k}_kib_per_launch={t[k]['kib_per_launch']:.1f}")
+            print(f"{k}_scale={t[k]['scale']:.5f}")
+            print(f"{k}_bytes_per_launch={t[k]['bytes_per_launch']:.0f}")
+        print(f"traffic_bytes_per_launch={t['bytes_per_launch']:.0f}")
+        return 0
+    print(__doc__, file=sys.stderr)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -7,8 +7,9 @@ Order of one force evaluation (sph_yamauchi/run/main.cpp:293-358):
   softening eps2, tree gravity, write-back (hdr_gravity.hpp:107-137, 346-423)
   hydro EPI prep, hydro, write-back, acc sum, alpha-dot
                                            (hdr_hydro.hpp, hdr_sph.hpp:95-97, hdr_hgas.hpp:750-762)
-The walk lists come from the host tree (tree.py) and are uploaded once per
-list build; they are inputs of the step, not part of it.
+The walk lists come from the device tree (sphk_dtree.h; tree="device") or
+the host tree (sphk_tree.h; tree="host"), which build identical lists; they
+are inputs of the step, not part of it.
 
 The density smoothing-length retry loop (hdr_density.hpp:369-415) is in
 `converge_density`: particles whose H_3 exceeds their search radius get
@@ -26,7 +27,7 @@ import torch
 from . import _lib as L
 from ._lib import check, lib
 from .device import FP64, MIXED, Context, Walks, to_device
-from .tree import HostTree, morton_order
+from .tree import DeviceTree, HostTree, morton_order, morton_order_dev
 
 KSRH = 1.936492
 
@@ -44,7 +45,10 @@ class SPHSystem:
     def __init__(self, p: dict, ctx: Context, eta: float = 1.6, eps: float | None = None,
                  theta: float = 0.5, n_group: int = 16, n_leaf: int = 8,
                  precision: int = MIXED, hmax: float | None = None, phys_params=None,
-                 device="cuda"):
+                 device="cuda", tree: str = "device"):
+        if tree not in ("device", "host"):
+            raise ValueError("tree must be 'device' or 'host'")
+        self.tree = tree
         self.ctx = ctx
         self.dev = device
         self.eta = eta
@@ -114,11 +118,16 @@ class SPHSystem:
 
     def resort(self):
         """Re-establish Morton order after particles moved (permutes the state)."""
-        x, y, z = self._host("x", "y", "z")
-        perm = morton_order(x, y, z)
-        if np.array_equal(perm, np.arange(self.n, dtype=perm.dtype)):
-            return
-        pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
+        if self.tree == "device":
+            pt = morton_order_dev(self.s["x"], self.s["y"], self.s["z"])
+            if bool((pt == torch.arange(self.n, device=pt.device)).all()):
+                return
+        else:
+            x, y, z = self._host("x", "y", "z")
+            perm = morton_order(x, y, z)
+            if np.array_equal(perm, np.arange(self.n, dtype=perm.dtype)):
+                return
+            pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
         for k in L.STATE_FIELDS:
             self.s[k].copy_(self.s[k][pt])
         for t in (self.id, self.istar, self.rs, self.np_, self.itr, self.eps2):
@@ -126,9 +135,14 @@ class SPHSystem:
 
     def build_lists(self, kinds=("dens", "hydro", "grav")):
         t0 = time.perf_counter()
-        x, y, z, m, ksr = self._host("x", "y", "z", "m", "ksr")
         if "grav" in kinds:
             self._softening()
+        if self.tree == "device":
+            self._build_lists_device(kinds)
+            torch.cuda.synchronize()
+            self.list_build_s = time.perf_counter() - t0
+            return
+        x, y, z, m, ksr = self._host("x", "y", "z", "m", "ksr")
         eps2 = self.eps2.cpu().numpy()
         trees = {}
 
@@ -154,6 +168,32 @@ class SPHSystem:
             t.close()
         torch.cuda.synchronize()
         self.list_build_s = time.perf_counter() - t0
+
+    def _build_lists_device(self, kinds):
+        """The same lists as the host tree, built on the GPU (sphk_dtree.h)."""
+        s = self.s
+        trees = {}
+
+        def tree_for(kind):
+            g = self.n_group[kind]
+            if g not in trees:
+                trees[g] = DeviceTree(s["x"], s["y"], s["z"], s["m"], self.eps2, self.n_leaf, g)
+            return trees[g]
+
+        try:
+            if "dens" in kinds:
+                t = tree_for("dens")
+                self.lists["dens"] = Walks(t.i_off, *t.gather(self.rs))
+            if "hydro" in kinds:
+                t = tree_for("hydro")
+                self.lists["hydro"] = Walks(t.i_off, *t.symmetry(s["ksr"]))
+            if "grav" in kinds:
+                t = tree_for("grav")
+                self.lists["grav"] = Walks(t.i_off, *t.long(self.theta))
+                self.mom = t.moments()
+        finally:
+            for t in trees.values():
+                t.close()
 
     def autotune_groups(self, candidates=(8, 16, 32), reps: int = 3, precision=None):
         """Pick, per list kind, the walk group size with the fastest kernel.

@@ -116,3 +116,111 @@ def brute_force_lists(n: int, x, y, z, group: int = 16):
     j_off = (np.arange(n_walk + 1, dtype=np.int64) * n).astype(np.int32)
     j_idx = np.tile(np.arange(n, dtype=np.int32), n_walk)
     return i_off, j_off, j_idx
+
+
+# --------------------------------------------------------------------------
+# Device-resident tree (include/sphk_dtree.h): same tree, walks and lists as
+# HostTree, built on the GPU from torch tensors.
+# --------------------------------------------------------------------------
+def _dptr(t):
+    import torch
+    if t is None:
+        return None
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+        raise ValueError("expected a contiguous CUDA tensor")
+    return t.data_ptr()
+
+
+def _stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _copy_out(ptr, n: int, dtype):
+    """Copy n elements at device address ptr into a new tensor (on the current stream)."""
+    import torch
+    out = torch.empty(n, dtype=dtype, device="cuda")
+    if n:
+        check(lib().sphk_copy_dev(out.data_ptr(), ptr, n * out.element_size(), _stream()),
+              "sphk_copy_dev")
+    return out
+
+
+def morton_order_dev(x, y, z):
+    """Device permutation into Morton order (int64 tensor), as morton_order."""
+    import torch
+    n = x.numel()
+    perm = torch.empty(n, dtype=torch.int32, device=x.device)
+    check(lib().sphk_morton_order_dev(n, _dptr(x), _dptr(y), _dptr(z), _dptr(perm), _stream()),
+          "sphk_morton_order_dev")
+    return perm.long()
+
+
+class DeviceTree:
+    """The octree, walks and lists of HostTree, built and held on the GPU.
+
+    Inputs are float64 CUDA tensors in Morton order; outputs are tensors on
+    the same device (copies owned by the caller)."""
+
+    def __init__(self, x, y, z, m, eps2=None, n_leaf: int = 8, n_group: int = 64):
+        import torch
+        h = _P()
+        n = x.numel()
+        check(lib().sphk_dtree_build(n, _dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(eps2),
+                                     int(n_leaf), int(n_group), _stream(), C.byref(h)),
+              "sphk_dtree_build")
+        self._h = h
+        self.n = n
+        self.n_walk = lib().sphk_dtree_n_walk(h)
+        self.n_node = lib().sphk_dtree_n_node(h)
+        self.i_off = _copy_out(lib().sphk_dtree_i_off(h), self.n_walk + 1, torch.int32)
+
+    def close(self):
+        if self._h:
+            lib().sphk_dtree_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _get(self, kind: int):
+        import torch
+        jo, ji, so, si = _P(), _P(), _P(), _P()
+        nj, ns = C.c_int64(), C.c_int64()
+        check(lib().sphk_dtree_list(self._h, kind, C.byref(jo), C.byref(ji), C.byref(so), C.byref(si),
+                                    C.byref(nj), C.byref(ns)), "sphk_dtree_list")
+        out = [_copy_out(jo.value, self.n_walk + 1, torch.int32), _copy_out(ji.value, nj.value, torch.int32)]
+        if kind == 2:
+            out += [_copy_out(so.value, self.n_walk + 1, torch.int32),
+                    _copy_out(si.value, ns.value, torch.int32)]
+        return tuple(out)
+
+    def gather(self, rs):
+        check(lib().sphk_dtree_list_gather(self._h, _dptr(rs)), "sphk_dtree_list_gather")
+        return self._get(0)
+
+    def symmetry(self, h):
+        check(lib().sphk_dtree_list_symmetry(self._h, _dptr(h)), "sphk_dtree_list_symmetry")
+        return self._get(1)
+
+    def long(self, theta: float):
+        check(lib().sphk_dtree_list_long(self._h, float(theta)), "sphk_dtree_list_long")
+        return self._get(2)
+
+    def moments(self) -> dict:
+        import torch
+        ps = [_P() for _ in range(5)]
+        check(lib().sphk_dtree_moments(self._h, *(C.byref(p) for p in ps)), "sphk_dtree_moments")
+        return {k: _copy_out(p.value, self.n_node, torch.float64)
+                for k, p in zip(("m", "x", "y", "z", "eps2"), ps)}
+
+    def nodes(self) -> dict:
+        import torch
+        ps = [_P() for _ in range(6)]
+        check(lib().sphk_dtree_nodes(self._h, *(C.byref(p) for p in ps)), "sphk_dtree_nodes")
+        names = ("begin", "end", "child", "nchild", "level", "len")
+        return {k: _copy_out(p.value, self.n_node, torch.float64 if k == "len" else torch.int32)
+                for k, p in zip(names, ps)}
