@@ -89,6 +89,16 @@ int sphk_correct_dev(sphk_ctx *ctx, int n, sphk_state s, double dt, void *stream
  *   out[5] total mass */
 int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *out, void *stream);
 
+/* Unbound mass of the current state (analysis/unbound_mass/unbound_mass.py:10-28,
+ * analysis/common.py:71-90): v_cg = sum rho v / sum rho over the selected
+ * particles, e_i = |v_i - v_cg|^2 / 2 + pot_i + u_i; hard = sum of m_i with
+ * e_i > 0, smoothed = sum m_i / (1 + exp(-k e_i)) (a differentiable
+ * surrogate; k in units of 1 / specific energy).  istar (device, may be NULL)
+ * with star >= 0 selects one star.  FP64 fixed-order reductions; out (host)
+ * = {v_cg x, y, z, hard, smoothed, selected mass} after a stream sync. */
+int sphk_unbound_mass_dev(sphk_ctx *ctx, int n, sphk_state s, const int32_t *istar, int star, double k,
+                          double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

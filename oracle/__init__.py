@@ -57,6 +57,8 @@ def lib():
         _lib.orc_timestep.restype = C.c_double
         _lib.orc_energy.restype = C.c_double
         _lib.orc_ksr_max.restype = C.c_double
+        _lib.orc_unbound_mass.restype = None
+        _lib.orc_unbound_mass.argtypes = [C.c_int, State, _P, C.c_int, C.c_double, C.POINTER(C.c_double)]
     return _lib
 
 
@@ -262,3 +264,11 @@ def ksr_max(m, x, y, z, dens, istar=None, istar_sel=0) -> float:
     arrs = [_f64(a) for a in (m, x, y, z, dens)]
     ist = None if istar is None else _i32(istar)
     return lib().orc_ksr_max(len(arrs[0]), *(_ptr(a) for a in arrs), _ptr(ist), int(istar_sel))
+
+
+def unbound_mass(s: dict, istar=None, star: int = -1, k: float = 20.0) -> np.ndarray:
+    """{v_cg (3), unbound mass, smoothed unbound mass, selected mass}."""
+    out = (C.c_double * 6)()
+    ist = None if istar is None else _i32(istar)
+    lib().orc_unbound_mass(len(s["m"]), _state(s), _ptr(ist), int(star), float(k), out)
+    return np.array(out[:])

@@ -453,3 +453,34 @@ double orc_ksr_max(int n, const double *m, const double *x, const double *y, con
     }
     return sqrt(r2sum / dnsum);
 }
+
+static double orc_sigmoid(double x)
+{
+    if (x >= 0.) return 1. / (1. + exp(-x));
+    const double e = exp(x);
+    return e / (1. + e);
+}
+
+void orc_unbound_mass(int n, orc_state s, const int32_t *istar, int star, double k, double out[6])
+{
+    double rho = 0., rvx = 0., rvy = 0., rvz = 0.;
+    for (int i = 0; i < n; i++) {
+        if (istar && star >= 0 && istar[i] != star) continue;
+        rho += s.dens[i];
+        rvx += s.dens[i] * s.vx[i];
+        rvy += s.dens[i] * s.vy[i];
+        rvz += s.dens[i] * s.vz[i];
+    }
+    const double cx = rvx / rho, cy = rvy / rho, cz = rvz / rho;
+    double hard = 0., soft = 0., mt = 0.;
+    for (int i = 0; i < n; i++) {
+        if (istar && star >= 0 && istar[i] != star) continue;
+        const double vx = s.vx[i] - cx, vy = s.vy[i] - cy, vz = s.vz[i] - cz;
+        const double e = 0.5 * (vx * vx + vy * vy + vz * vz) + s.pot[i] + s.uene[i];
+        if (e > 0.) hard += s.m[i];
+        soft += s.m[i] * orc_sigmoid(k * e);
+        mt += s.m[i];
+    }
+    out[0] = cx; out[1] = cy; out[2] = cz;
+    out[3] = hard; out[4] = soft; out[5] = mt;
+}

@@ -151,6 +151,15 @@ void orc_momentum(int n, orc_state s, double out[3]);
 double orc_ksr_max(int n, const double *m, const double *x, const double *y, const double *z,
                    const double *dens, const int32_t *istar, int istar_sel);
 
+/* Unbound mass of one snapshot, analysis/unbound_mass/unbound_mass.py:10-28:
+ * v_cg = sum rho v / sum rho over the selected particles
+ * (analysis/common.py:71-90), e_i = |v_i - v_cg|^2 / 2 + pot_i + u_i, and
+ * the mass with e_i > 0.  Also the smoothed mass sum m sigma(k e_i),
+ * sigma(x) = 1 / (1 + exp(-x)).  Selection: istar == NULL or star < 0 takes
+ * every particle, else those with istar[i] == star.
+ * out = {v_cg x, y, z, unbound (hard), unbound (smoothed), selected mass}. */
+void orc_unbound_mass(int n, orc_state s, const int32_t *istar, int star, double k, double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

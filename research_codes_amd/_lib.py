@@ -131,6 +131,7 @@ SIGNATURES.update({
     "sphk_dtree_moments": (C.c_int, [_P] + [C.POINTER(_P)] * 5),
     "sphk_dtree_nodes": (C.c_int, [_P] + [C.POINTER(_P)] * 6),
     "sphk_copy_dev": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "sphk_unbound_mass_dev": (C.c_int, [_P, C.c_int, State, _P, C.c_int, C.c_double, _DP, _P]),
 })
 
 _lib = None

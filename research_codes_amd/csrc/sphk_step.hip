@@ -170,7 +170,9 @@ __device__ __forceinline__ void diag_terms(const sphk_state &s, const sphk_phys 
     v[5] = m;
 }
 
-__device__ __forceinline__ void block_reduce6(double v[6], double *sh)
+// Block reduction of six values: sums, except the minimum for the slots in
+// min_mask (bit k = slot k).
+__device__ __forceinline__ void block_reduce6(double v[6], double *sh, int min_mask = 1 << 4)
 {
     const int t = threadIdx.x;
     for (int k = 0; k < 6; k++) sh[k * kBlock + t] = v[k];
@@ -179,7 +181,7 @@ __device__ __forceinline__ void block_reduce6(double v[6], double *sh)
         if (t < w) {
             for (int k = 0; k < 6; k++) {
                 const double a = sh[k * kBlock + t], b = sh[k * kBlock + t + w];
-                sh[k * kBlock + t] = (k == 4) ? (a < b ? a : b) : a + b;
+                sh[k * kBlock + t] = ((min_mask >> k) & 1) ? (a < b ? a : b) : a + b;
             }
         }
         __syncthreads();
@@ -250,6 +252,72 @@ int pre(sphk_ctx *ctx, int n, const sphk_state *s)
     return SPHK_OK;
 }
 
+// ---- unbound mass (two fixed-order reductions) ---------------------------
+__device__ __forceinline__ bool selected(const int32_t *istar, int star, int i)
+{
+    return istar == nullptr || star < 0 || istar[i] == star;
+}
+
+// pass 1: sum rho, rho v (slots 0..3)
+__global__ __launch_bounds__(kBlock) void unbound_vcg_partial(int n, sphk_state s, const int32_t *istar,
+                                                              int star, double *part)
+{
+    __shared__ double sh[6 * kBlock];
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        if (!selected(istar, star, i)) continue;
+        const double r = s.dens[i];
+        acc[0] += r;
+        acc[1] += r * s.vx[i];
+        acc[2] += r * s.vy[i];
+        acc[3] += r * s.vz[i];
+    }
+    block_reduce6(acc, sh, 0);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) part[k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+__device__ __forceinline__ double sigmoid(double x)
+{
+    if (x >= 0.) return 1. / (1. + exp(-x));
+    const double e = exp(x);
+    return e / (1. + e);
+}
+
+// pass 2: hard and smoothed unbound mass, selected mass (slots 0..2);
+// vcg = the pass-1 totals
+__global__ __launch_bounds__(kBlock) void unbound_mass_partial(int n, sphk_state s, const int32_t *istar,
+                                                               int star, double k, const double *vcg,
+                                                               double *part)
+{
+    __shared__ double sh[6 * kBlock];
+    const double cx = vcg[1] / vcg[0], cy = vcg[2] / vcg[0], cz = vcg[3] / vcg[0];
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        if (!selected(istar, star, i)) continue;
+        const double vx = s.vx[i] - cx, vy = s.vy[i] - cy, vz = s.vz[i] - cz;
+        const double e = 0.5 * (vx * vx + vy * vy + vz * vz) + s.pot[i] + s.uene[i];
+        const double m = s.m[i];
+        acc[0] += e > 0. ? m : 0.;
+        acc[1] += m * sigmoid(k * e);
+        acc[2] += m;
+    }
+    block_reduce6(acc, sh, 0);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; q++) part[q * gridDim.x + blockIdx.x] = acc[q];
+}
+
+__global__ __launch_bounds__(kBlock) void sum_final_kernel(int nb, const double *part, double *out)
+{
+    __shared__ double sh[6 * kBlock];
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    for (int b = threadIdx.x; b < nb; b += kBlock)
+        for (int k = 0; k < 6; k++) acc[k] += part[k * nb + b];
+    block_reduce6(acc, sh, 0);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) out[k] = acc[k];
+}
+
 // Scratch for the diagnostics partials: one per process and device ordinal.
 struct DiagScratch {
     double *part = nullptr;
@@ -258,6 +326,21 @@ struct DiagScratch {
     int device = -1;
 };
 thread_local DiagScratch g_diag;
+thread_local DiagScratch g_unbound;   // out holds 12: pass-1 and pass-2 totals
+
+int diag_scratch(DiagScratch &d, int dev, int nb, int nout)
+{
+    if (d.device == dev && d.cap >= nb) return SPHK_OK;
+    if (d.part) (void)hipFree(d.part);
+    if (d.out) (void)hipFree(d.out);
+    d = DiagScratch();
+    if (hipMalloc(&d.part, sizeof(double) * 6 * nb) != hipSuccess ||
+        hipMalloc(&d.out, sizeof(double) * nout) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "diagnostics: hipMalloc failed");
+    d.cap = nb;
+    d.device = dev;
+    return SPHK_OK;
+}
 
 }  // namespace
 
@@ -359,6 +442,34 @@ int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *ou
     if (hipMemcpyAsync(out, g_diag.out, sizeof(double) * 6, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return set_error(SPHK_E_DEVICE, "diagnostics: copy-back failed");
+    return SPHK_OK;
+}
+
+int sphk_unbound_mass_dev(sphk_ctx *ctx, int n, sphk_state s, const int32_t *istar, int star, double k,
+                          double *out, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc) return rc;
+    if (!out) return set_error(SPHK_E_ARG, "unbound_mass: null output");
+    const int nb = 1024;
+    if ((rc = diag_scratch(g_unbound, sphk_device(ctx), nb, 12))) return rc;
+    hipStream_t st = stream_of(ctx, stream);
+    hipLaunchKernelGGL(unbound_vcg_partial, dim3(nb), dim3(kBlock), 0, st, n, s, istar, star, g_unbound.part);
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(kBlock), 0, st, nb, g_unbound.part, g_unbound.out);
+    hipLaunchKernelGGL(unbound_mass_partial, dim3(nb), dim3(kBlock), 0, st, n, s, istar, star, k, g_unbound.out,
+                       g_unbound.part);
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(kBlock), 0, st, nb, g_unbound.part, g_unbound.out + 6);
+    if ((rc = launch_check("unbound_mass"))) return rc;
+    double h[12];
+    if (hipMemcpyAsync(h, g_unbound.out, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "unbound_mass: copy-back failed");
+    out[0] = h[1] / h[0];
+    out[1] = h[2] / h[0];
+    out[2] = h[3] / h[0];
+    out[3] = h[6];
+    out[4] = h[7];
+    out[5] = h[8];
     return SPHK_OK;
 }
 

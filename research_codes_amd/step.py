@@ -345,6 +345,25 @@ class SPHSystem:
               "sphk_diagnostics")
         return dict(energy=out[0], momentum=np.array(out[1:4]), dt_min=out[4], mass=out[5])
 
+    def snapshot(self) -> dict:
+        """Host copy of the particle state in id order (for the writers in
+        research_codes_amd.formats)."""
+        order = torch.argsort(self.id.long())
+        p = {k: v[order].cpu().numpy() for k, v in self.s.items()}
+        p["id"] = self.id[order].cpu().numpy().astype(np.int64)
+        p["istar"] = self.istar[order].cpu().numpy().astype(np.int64)
+        p["np"] = self.np_[order].cpu().numpy()
+        return p
+
+    def unbound_mass(self, k: float = 20.0, star: int = -1) -> dict:
+        """Unbound mass (analysis/unbound_mass/unbound_mass.py:10-28) of the
+        current state, hard threshold and smoothed with slope k; star >= 0
+        restricts to one star."""
+        out = (C.c_double * 6)()
+        check(lib().sphk_unbound_mass_dev(self.ctx.handle, self.n, self._ss, self.istar.data_ptr(),
+                                          int(star), float(k), out, self._st()), "sphk_unbound_mass_dev")
+        return dict(v_cg=np.array(out[0:3]), unbound=out[3], unbound_smooth=out[4], mass=out[5])
+
     def next_dt(self, dt_min_particle: float, dt_floor: float = 1e-16) -> float:
         """Power-of-two timestep adjustment of main.cpp:241-273."""
         dtc = min(self.phys.dt_max, dt_min_particle)
