@@ -220,8 +220,12 @@ def test_run_driver_round_trip(tmp_path):
     res = run(tmp_path / "two.head", tmp_path / "two.data", tmp_path / "out", max_steps=3)
     rows = formats.read_time_log(tmp_path / "out" / "time.log")
     assert rows.shape[0] == 3 and list(rows[:, 0]) == [0, 1, 2]
+    # the first step relaxes H from the sampled initial condition (eps = H/kappa
+    # changes with it, and the reference drops the d eps / dH term): energy
+    # moves by ~1e-3 there and by far less afterwards
     e = rows[:, 4]
-    assert np.all(np.abs(e / e[0] - 1) < 1e-3)
+    assert np.all(np.abs(e / e[0] - 1) < 1e-2)
+    assert abs(e[2] / e[1] - 1) < 1e-4
     snap0 = formats.read_snapshot(tmp_path / "out" / "sph_t000000.dat")
     assert np.array_equal(snap0["id"], np.sort(p["id"]))
     # positions on disk are cgs; read back in reference code units
