@@ -179,3 +179,33 @@ def off_axis(N_total: int = 1_000_000, eta: float = 1.18) -> dict:
     a = polytrope(3.0, N0, 1.0, 1.0, seed=3000, eta=eta)
     b = polytrope(1.5, N1, 0.5, 0.6, seed=3001, eta=eta)
     return collision(a, b, 3.0, 0.25)
+
+
+def ensemble(n_runs: int = 64, n_min: float = 1e4, n_max: float = 2e5, seed: int = 4000,
+             eta: float = 1.18) -> tuple[dict, list[int], list[dict]]:
+    """Independent collisions for batched evaluation.
+
+    Run r: N_r = exp(U(ln n_min, ln n_max)) particles, impactor mass ratio
+    q ~ U(0.3, 1) with radius q^(-1/3) (the n=1.5 mass-radius relation
+    R ~ M^(-1/3)), impact parameter b ~ U(0, 1) at separation 3, both stars
+    n=1.5 and equal particle mass.  Returns the particles of all runs
+    concatenated (ids unique across runs), the particle count of each run,
+    and the per-run parameters."""
+    u = uniform53(seed, 3 * n_runs).reshape(n_runs, 3)
+    le = LaneEmden(1.5)
+    runs, counts, meta, id0 = [], [], [], 0
+    for r in range(n_runs):
+        n = int(round(math.exp(math.log(n_min) + u[r, 0] * (math.log(n_max) - math.log(n_min)))))
+        q = 0.3 + 0.7 * u[r, 1]
+        b = u[r, 2]
+        n0 = max(2, int(round(n / (1.0 + q))))
+        n1 = max(2, n - n0)
+        a = polytrope(1.5, n0, 1.0, 1.0, seed=seed + 2 * r + 1, eta=eta, le=le)
+        c = polytrope(1.5, n1, q, q ** (-1.0 / 3.0), seed=seed + 2 * r + 2, eta=eta, le=le)
+        p = collision(a, c, 3.0, b)
+        p["id"] = p["id"] + id0
+        id0 += len(p["id"])
+        runs.append(p)
+        counts.append(len(p["id"]))
+        meta.append({"n": len(p["id"]), "q": q, "b": b})
+    return concat(*runs), counts, meta

@@ -45,7 +45,7 @@ class SPHSystem:
     def __init__(self, p: dict, ctx: Context, eta: float = 1.6, eps: float | None = None,
                  theta: float = 0.5, n_group: int = 16, n_leaf: int = 8,
                  precision: int = MIXED, hmax: float | None = None, phys_params=None,
-                 device="cuda", tree: str = "device"):
+                 device="cuda", tree: str = "device", segments=None):
         if tree not in ("device", "host"):
             raise ValueError("tree must be 'device' or 'host'")
         self.tree = tree
@@ -60,7 +60,15 @@ class SPHSystem:
         self.n_leaf = n_leaf
         self.precision = precision
         self.phys = phys_params or phys()
-        perm = morton_order(p["x"], p["y"], p["z"])
+        # independent systems (an ensemble of runs): particle counts of
+        # consecutive blocks of p; each is Morton-sorted and gets its own tree
+        n_all = len(p["x"])
+        counts = [n_all] if segments is None else [int(c) for c in segments]
+        if sum(counts) != n_all or min(counts) < 1:
+            raise ValueError("segments must be positive counts summing to the particle number")
+        self.seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        perm = np.concatenate([morton_order(p["x"][a:b], p["y"][a:b], p["z"][a:b]) + a
+                               for a, b in zip(self.seg[:-1], self.seg[1:])])
         self.n = n = len(perm)
         f64 = dict(dtype=torch.float64, device=device)
         self.s = {k: torch.zeros(n, **f64) for k in L.STATE_FIELDS}
@@ -79,7 +87,7 @@ class SPHSystem:
         self.time = 0.0
         self.dt = self.phys.dt_max
         if hmax is None:
-            hmax = self.system_size()
+            hmax = self.system_size() if len(counts) == 1 else 1e30
         # a constant softening makes every eps2 equal: one rsqrt per gravity pair
         ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax, uniform_softening=eps is not None)
         self.lists = {}
@@ -116,15 +124,20 @@ class SPHSystem:
     def _host(self, *keys):
         return [self.s[k].cpu().numpy() for k in keys]
 
+    def _segments(self):
+        return list(zip(self.seg[:-1].tolist(), self.seg[1:].tolist()))
+
     def resort(self):
-        """Re-establish Morton order after particles moved (permutes the state)."""
+        """Re-establish Morton order after particles moved (permutes the state;
+        each segment of an ensemble on its own)."""
         if self.tree == "device":
-            pt = morton_order_dev(self.s["x"], self.s["y"], self.s["z"])
+            x, y, z = self.s["x"], self.s["y"], self.s["z"]
+            pt = torch.cat([morton_order_dev(x[a:b], y[a:b], z[a:b]) + a for a, b in self._segments()])
             if bool((pt == torch.arange(self.n, device=pt.device)).all()):
                 return
         else:
             x, y, z = self._host("x", "y", "z")
-            perm = morton_order(x, y, z)
+            perm = np.concatenate([morton_order(x[a:b], y[a:b], z[a:b]) + a for a, b in self._segments()])
             if np.array_equal(perm, np.arange(self.n, dtype=perm.dtype)):
                 return
             pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
@@ -137,63 +150,82 @@ class SPHSystem:
         t0 = time.perf_counter()
         if "grav" in kinds:
             self._softening()
-        if self.tree == "device":
-            self._build_lists_device(kinds)
-            torch.cuda.synchronize()
-            self.list_build_s = time.perf_counter() - t0
-            return
-        x, y, z, m, ksr = self._host("x", "y", "z", "m", "ksr")
-        eps2 = self.eps2.cpu().numpy()
-        trees = {}
-
-        def tree_for(kind):
-            g = self.n_group[kind]
-            if g not in trees:
-                trees[g] = HostTree(x, y, z, m, eps2, self.n_leaf, g)
-            return trees[g]
-
-        if "dens" in kinds:
-            rs = self.rs.cpu().numpy()
-            t = tree_for("dens")
-            self.lists["dens"] = Walks(t.i_off, *t.gather(rs), device=self.dev)
-        if "hydro" in kinds:
-            t = tree_for("hydro")
-            self.lists["hydro"] = Walks(t.i_off, *t.symmetry(ksr), device=self.dev)
-        if "grav" in kinds:
-            t = tree_for("grav")
-            j_off, j_idx, s_off, s_idx = t.long(self.theta)
-            self.lists["grav"] = Walks(t.i_off, j_off, j_idx, s_off, s_idx, device=self.dev)
-            self.mom = {k: to_device(v, self.dev) for k, v in t.moments().items()}
-        for t in trees.values():
-            t.close()
+        self._build_lists_segments(kinds)
         torch.cuda.synchronize()
         self.list_build_s = time.perf_counter() - t0
 
-    def _build_lists_device(self, kinds):
-        """The same lists as the host tree, built on the GPU (sphk_dtree.h)."""
+    def _tree(self, a: int, b: int, g: int):
+        """Tree over particles [a, b) with walk group size g."""
         s = self.s
-        trees = {}
+        if self.tree == "device":
+            return DeviceTree(s["x"][a:b], s["y"][a:b], s["z"][a:b], s["m"][a:b], self.eps2[a:b],
+                              self.n_leaf, g)
+        x, y, z, m = (s[k][a:b].cpu().numpy() for k in ("x", "y", "z", "m"))
+        return HostTree(x, y, z, m, self.eps2[a:b].cpu().numpy(), self.n_leaf, g)
 
-        def tree_for(kind):
-            g = self.n_group[kind]
-            if g not in trees:
-                trees[g] = DeviceTree(s["x"], s["y"], s["z"], s["m"], self.eps2, self.n_leaf, g)
-            return trees[g]
+    def _build_lists_segments(self, kinds):
+        """Lists per segment (one tree each, device or host), their indices
+        shifted to the segment's place and concatenated into one walk table
+        per functor: an ensemble runs in one launch per functor."""
+        s = self.s
+        dev = torch.device(self.dev)
 
-        try:
-            if "dens" in kinds:
-                t = tree_for("dens")
-                self.lists["dens"] = Walks(t.i_off, *t.gather(self.rs))
-            if "hydro" in kinds:
-                t = tree_for("hydro")
-                self.lists["hydro"] = Walks(t.i_off, *t.symmetry(s["ksr"]))
-            if "grav" in kinds:
-                t = tree_for("grav")
-                self.lists["grav"] = Walks(t.i_off, *t.long(self.theta))
-                self.mom = t.moments()
-        finally:
-            for t in trees.values():
-                t.close()
+        def t32(a):
+            return a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+        parts = {k: [] for k in kinds}
+        moms = []
+        for a, b in self._segments():
+            trees = {}
+            try:
+                def tree_for(kind):
+                    g = self.n_group[kind]
+                    if g not in trees:
+                        trees[g] = self._tree(a, b, g)
+                    return trees[g]
+                dev_tree = self.tree == "device"
+                if "dens" in kinds:
+                    t = tree_for("dens")
+                    rs = self.rs[a:b] if dev_tree else self.rs[a:b].cpu().numpy()
+                    parts["dens"].append((a, t.i_off, *t.gather(rs)))
+                if "hydro" in kinds:
+                    t = tree_for("hydro")
+                    h = s["ksr"][a:b] if dev_tree else s["ksr"][a:b].cpu().numpy()
+                    parts["hydro"].append((a, t.i_off, *t.symmetry(h)))
+                if "grav" in kinds:
+                    t = tree_for("grav")
+                    parts["grav"].append((a, t.i_off, *t.long(self.theta)))
+                    moms.append({k: t32(v) for k, v in t.moments().items()})
+            finally:
+                for t in trees.values():
+                    t.close()
+        for kind, pl in parts.items():
+            if len(pl) == 1:
+                a, i_off, *rest = pl[0]
+                self.lists[kind] = Walks(t32(i_off), *map(t32, rest))
+                continue
+            i_off, j_off, j_idx, s_off, s_idx = [], [], [], [], []
+            jb = sb = nb = 0
+            for (a, io, *rest), mom in zip(pl, moms if kind == "grav" else [None] * len(pl)):
+                io = t32(io)
+                jo, ji = t32(rest[0]), t32(rest[1])
+                i_off.append(io[:-1] + a)
+                j_off.append(jo[:-1] + jb)
+                j_idx.append(ji + a)
+                jb += int(jo[-1])
+                if kind == "grav":
+                    so, si = t32(rest[2]), t32(rest[3])
+                    s_off.append(so[:-1] + sb)
+                    s_idx.append(si + nb)
+                    sb += int(so[-1])
+                    nb += int(mom["m"].numel())
+            tail = lambda v: torch.tensor([v], dtype=torch.int32, device=dev)   # noqa: E731
+            args = [torch.cat(i_off + [tail(self.n)]), torch.cat(j_off + [tail(jb)]), torch.cat(j_idx)]
+            if kind == "grav":
+                args += [torch.cat(s_off + [tail(sb)]), torch.cat(s_idx)]
+            self.lists[kind] = Walks(*args)
+        if "grav" in kinds:
+            self.mom = moms[0] if len(moms) == 1 else {k: torch.cat([m[k] for m in moms]) for k in moms[0]}
 
     def autotune_groups(self, candidates=(8, 16, 32), reps: int = 3, precision=None):
         """Pick, per list kind, the walk group size with the fastest kernel.
