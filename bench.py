@@ -69,11 +69,19 @@ def main() -> int:
                     help="walk group sizes dens,hydro,grav (e.g. 16,16,32); implies --no-autotune")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run 3 plain 256 MiB device copies (counter scale reference)")
+    ap.add_argument("--workload", choices=("collision", "ensemble", "drift"), default="collision",
+                    help="collision: the N=1e6 off-axis step (default); ensemble: 64 independent "
+                         "collisions (sum N ~ 4.3e6) in one launch per functor; drift: 2e5-particle "
+                         "head-on collision, --steps KDK steps in mixed and in FP64")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU spatial decomposition "
               "is a design note, not code); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.workload == "ensemble":
+        return ensemble_main(args)
+    if args.workload == "drift":
+        return drift_main(args)
 
     import torch
     from research_codes_amd import ic
@@ -223,6 +231,135 @@ def main() -> int:
     out.update(res)
     print(json.dumps(out), flush=True)
     ctx.close()
+    return 0
+
+
+def ensemble_main(args) -> int:
+    """64 collisions of 1e4..2e5 particles (log-uniform), q ~ U(0.3, 1),
+    b ~ U(0, 1): one force evaluation over all runs with one launch per
+    functor, against the same runs launched one after another."""
+    import torch
+    from research_codes_amd import ic
+    from research_codes_amd.device import MIXED, Context
+    from research_codes_amd.step import SPHSystem
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    p, counts, meta = ic.ensemble(64, eta=args.eta)
+    ctx = Context(0)
+    ctx.timing(True)
+    g = 32 if args.groups is None else int(args.groups.split(",")[0])
+    s = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=g, precision=MIXED, hmax=1e30,
+                  segments=counts)
+    t0 = time.perf_counter()
+    s.relax_h(2)
+    s.eos_balsara()
+    s.rs.copy_(1.1 * s.s["ksr"])
+    s.build_lists(("dens",))
+    snap = {k: v.clone() for k, v in s.s.items()}
+    s.density()
+    s.eos_balsara()
+    s.build_lists(("hydro", "grav"))
+    log(f"ensemble of {len(counts)} runs, N={s.n}, set-up {time.perf_counter() - t0:.1f}s")
+    entries = sum(int(w.entries + w.entries_spj) for w in s.lists.values())
+
+    def timed(fn, steps):
+        for k, v in snap.items():
+            s.s[k].copy_(v)
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            for k, v in snap.items():
+                s.s[k].copy_(v)
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps * 1e3
+
+    ms_batched = timed(lambda: s.force(MIXED), args.steps)
+    # the same walks launched run by run (one launch per functor per run)
+    full = dict(s.lists)
+    per_run = []
+    for a, b in s._segments():
+        sub = {}
+        for k, w in full.items():
+            lo = int(torch.searchsorted(w.i_off, torch.tensor([a], dtype=torch.int32, device=w.i_off.device)))
+            hi = int(torch.searchsorted(w.i_off, torch.tensor([b], dtype=torch.int32, device=w.i_off.device)))
+            from research_codes_amd.device import Walks
+            sub[k] = Walks(w.i_off[lo:hi + 1].contiguous(), w.j_off[lo:hi + 1].contiguous(), w.j_idx,
+                           None if w.s_off is None else w.s_off[lo:hi + 1].contiguous(), w.s_idx)
+        per_run.append(sub)
+
+    def sequential():
+        for sub in per_run:
+            s.lists = sub
+            s.density()
+        s.eos_balsara()
+        for sub in per_run:
+            s.lists = sub
+            s.gravity()
+        for sub in per_run:
+            s.lists = sub
+            s.hydro()
+        s.lists = full
+
+    ms_seq = timed(sequential, max(1, args.steps // 2))
+    out = {"metric": "ensemble force evaluation, interactions/s (64 collisions, one launch per functor)",
+           "value": entries / (ms_batched * 1e-3), "unit": "interactions/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 1, "ms_per_step": round(ms_batched, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic (seeded Lane-Emden polytropes)",
+           "config": {"workload": "ensemble of 64 parabolic collisions, N_run log-uniform in [1e4, 2e5], "
+                                  "q ~ U(0.3,1), b ~ U(0,1)", "N": s.n, "runs": len(counts), "n_group": g},
+           "ms_per_step_run_by_run": round(ms_seq, 4), "speedup_batched": round(ms_seq / ms_batched, 3),
+           "entries": entries}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+def drift_main(args) -> int:
+    """Head-on parabolic collision of two n=1.5 stars, 1e5 particles each,
+    constant softening 0.01: --steps KDK steps (full list rebuild and density
+    iteration every step) in mixed precision and in FP64 from the same
+    initial state; energy drift of both and the mixed-vs-FP64 difference."""
+    import torch
+    from research_codes_amd import ic
+    from research_codes_amd.device import FP64, MIXED, Context
+    from research_codes_amd.step import SPHSystem
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    p = ic.head_on(100_000, eta=args.eta)
+    res = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        with Context(0) as ctx:
+            s = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=32, precision=prec)
+            s.initialize()
+            e0 = s.diagnostics()["energy"]
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for k in range(args.steps):
+                s.kdk()
+                if (k + 1) % 20 == 0:
+                    log(f"{name}: step {k + 1}, t={s.time:.4g}")
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t
+            e1 = s.diagnostics()["energy"]
+            snap = s.snapshot()
+            res[name] = {"drift": abs(e1 / e0 - 1), "ms_per_step": wall / args.steps * 1e3, "time": s.time,
+                         "pos": np.stack([snap[c] for c in ("x", "y", "z")], 1)}
+    dpos = float(np.abs(res["mixed"]["pos"] - res["fp64"]["pos"]).max())
+    out = {"metric": "KDK energy drift, mixed vs FP64 (2x1e5 head-on collision)",
+           "value": res["mixed"]["drift"], "unit": "|E_K/E_0 - 1|", "n_gpus": 1, "steps": args.steps,
+           "warmup": 0, "ms_per_step": round(res["mixed"]["ms_per_step"], 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic (seeded Lane-Emden polytropes)",
+           "config": {"workload": "head-on parabolic collision of two n=1.5 stars (M=R=1), 2x1e5 particles, "
+                                  "eps=0.01, theta=0.5", "N": 200_000, "eta": args.eta},
+           "drift_fp64": res["fp64"]["drift"], "drift_ratio_mixed_over_fp64":
+               res["mixed"]["drift"] / max(res["fp64"]["drift"], 1e-300),
+           "ms_per_step_fp64": round(res["fp64"]["ms_per_step"], 3), "t_end": res["mixed"]["time"],
+           "same_t_end": res["mixed"]["time"] == res["fp64"]["time"], "max_abs_pos_diff": dpos}
+    print(json.dumps(out), flush=True)
     return 0
 
 
