@@ -14,10 +14,10 @@
 //      contraction off, so the FP64 sums round as on the host.
 //   4. Groups: a node is a walk if it holds <= n_group particles (or is a
 //      leaf) and its parent holds more; walks are ordered by first particle.
-//   5. Lists: one thread per walk runs the host's depth-first traversal
-//      (children pushed in reverse, popped in order) twice: count, then
-//      fill at scanned offsets; the output is the host's, element for
-//      element.
+//   5. Lists: the host's depth-first traversal of each walk is split at a
+//      frontier of subtrees (one tree level plus the leaves above it), one
+//      thread per (walk, subtree), run twice: count, then fill at scanned
+//      offsets; the output is the host's, element for element.
 #include <hip/hip_runtime.h>
 
 #include <cstring>   // rocPRIM's iterator headers use memset without including it
@@ -430,139 +430,203 @@ __device__ __forceinline__ bool box_overlaps(const double (&alo)[3], const doubl
     return true;
 }
 
-// Depth-first walk of one group; MODE 0 gather, 1 symmetry, 2 long.  With
-// FILL the EPJ / SPJ entries are written from jo / so, else counted.
+// Box of the i search spheres of each walk (gather: rs, symmetry: h).
+__global__ __launch_bounds__(kBlock) void walk_box_kernel(int ng, const int32_t *__restrict__ i_off,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ z,
+                                                          const double *__restrict__ r, double *__restrict__ wb)
+{
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ng) return;
+    const double inf = __builtin_inf();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (int i = i_off[g]; i < i_off[g + 1]; i++) {
+        const double p[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], p[d] - r[i]);
+            hi[d] = fmax(hi[d], p[d] + r[i]);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        wb[d * ng + g] = lo[d];
+        wb[(3 + d) * ng + g] = hi[d];
+    }
+}
+
+// What the depth-first walk does at node a: skip it (pruned, or massless for
+// the long list), take it as one SPJ, emit its particles (leaf), or open it.
+enum Visit { kSkip, kSpj, kLeaf, kOpen };
+
+template <int MODE>
+__device__ __forceinline__ Visit visit(const TreeView &t, size_t C, int a, const double (&olo)[3],
+                                       const double (&ohi)[3], const double (&glo)[3], const double (&ghi)[3],
+                                       double inv_t2)
+{
+#pragma clang fp contract(off)
+    const bool leaf = t.nd.child[a] < 0;
+    if (MODE == 0) {
+        if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a)) return kSkip;
+        return leaf ? kLeaf : kOpen;
+    }
+    if (MODE == 1) {
+        if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a) && !box_overlaps(glo, ghi, t.olo, t.ohi, C, a))
+            return kSkip;
+        return leaf ? kLeaf : kOpen;
+    }
+    if (t.mo.mm[a] == 0.) return kSkip;
+    const double c[3] = {t.mo.mx[a], t.mo.my[a], t.mo.mz[a]};
+    double s = 0.;
+    for (int d = 0; d < 3; d++) {
+        double e = 0.;
+        if (c[d] < glo[d])
+            e = glo[d] - c[d];
+        else if (c[d] > ghi[d])
+            e = c[d] - ghi[d];
+        s += e * e;
+    }
+    const double L = t.nd.len[a];
+    if (s > L * L * inv_t2) return kSpj;
+    return leaf ? kLeaf : kOpen;
+}
+
+// The host builder's depth-first walk of one group, split at the frontier:
+// frontier node f (the nodes of one level plus the leaves above it, in
+// particle order) gets its own thread per walk.  The thread first replays
+// the walk's decisions on the ancestors of f, root first: if one is skipped
+// nothing is emitted; if one is taken as an SPJ it is emitted by the thread
+// of its first frontier node only.  Otherwise it walks the subtree of f.
+// Concatenated in frontier order, the threads' outputs are the host's list.
+// MODE 0 gather, 1 symmetry, 2 long; with FILL entries are written from the
+// scanned offsets, else counted.
 template <int MODE, bool FILL>
-__global__ __launch_bounds__(64) void list_kernel(int ng, TreeView t, const double *__restrict__ r,
+__global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, const int32_t *__restrict__ front,
+                                                  const double *__restrict__ wb, const double *__restrict__ r,
                                                   double inv_t2, int64_t *__restrict__ cj,
-                                                  int64_t *__restrict__ cs, const int32_t *__restrict__ jo,
-                                                  const int32_t *__restrict__ so, int32_t *__restrict__ jout,
+                                                  int64_t *__restrict__ cs, const int64_t *__restrict__ oj,
+                                                  const int64_t *__restrict__ os, int32_t *__restrict__ jout,
                                                   int32_t *__restrict__ sout, int *__restrict__ err)
 {
 #pragma clang fp contract(off)
-    const int g = blockIdx.x * 64 + threadIdx.x;
-    if (g >= ng) return;
+    // neighbouring lanes take neighbouring walks under the same frontier
+    // node: their traversals follow nearly the same path
+    const int64_t t0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (t0 >= (int64_t)ng * nf) return;
+    const int g = (int)(t0 % ng), k = (int)(t0 / ng);
+    const int64_t tid = (int64_t)g * nf + k;   // slot of (walk, frontier node) in the count arrays
+    const int f = front[k];
     const size_t C = t.mo.cap;
     const int ib = t.i_off[g], ie = t.i_off[g + 1];
     const int gn = t.group[g];
-    const double inf = __builtin_inf();
-    // box of the i search spheres (gather: rs; symmetry: h); group tight box
-    double olo[3] = {inf, inf, inf}, ohi[3] = {-inf, -inf, -inf};
-    if (MODE != 2)
-        for (int i = ib; i < ie; i++) {
-            const double p[3] = {t.x[i], t.y[i], t.z[i]};
-            for (int d = 0; d < 3; d++) {
-                olo[d] = fmin(olo[d], p[d] - r[i]);
-                ohi[d] = fmax(ohi[d], p[d] + r[i]);
-            }
-        }
-    double glo[3], ghi[3];
+    double olo[3] = {0., 0., 0.}, ohi[3] = {0., 0., 0.}, glo[3], ghi[3];
     for (int d = 0; d < 3; d++) {
+        if (MODE != 2) {
+            olo[d] = wb[d * ng + g];
+            ohi[d] = wb[(3 + d) * ng + g];
+        }
         glo[d] = t.mo.lo[d * C + gn];
         ghi[d] = t.mo.hi[d * C + gn];
     }
     int64_t nj = 0, ns = 0;
-    const int64_t j0 = FILL ? (int64_t)jo[g] : 0;
-    const int64_t s0 = (FILL && MODE == 2) ? (int64_t)so[g] : 0;
-    int32_t st[kStack];
-    int sp = 0;
-    st[sp++] = 0;
-    while (sp > 0) {
-        const int a = st[--sp];
-        const int c0 = t.nd.child[a], nc = t.nd.nchild[a];
-        bool open = false;
-        if (MODE == 0) {
-            if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a)) continue;
-            if (c0 < 0) {
-                for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
-                    const double p[3] = {t.x[j], t.y[j], t.z[j]};
-                    bool in = true;
-                    for (int d = 0; d < 3; d++) in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
-                    if (!in) continue;
-                    for (int i = ib; i < ie; i++) {
-                        const double dx = p[0] - t.x[i], dy = p[1] - t.y[i], dz = p[2] - t.z[i];
-                        if (dx * dx + dy * dy + dz * dz < r[i] * r[i]) {
-                            if (FILL) jout[j0 + nj] = j;
-                            nj++;
-                            break;
-                        }
-                    }
-                }
-            } else {
-                open = true;
-            }
-        } else if (MODE == 1) {
-            if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a) && !box_overlaps(glo, ghi, t.olo, t.ohi, C, a))
-                continue;
-            if (c0 < 0) {
-                for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
-                    const double p[3] = {t.x[j], t.y[j], t.z[j]};
-                    const double hj = r[j];
-                    bool in = true, jov = true;
-                    for (int d = 0; d < 3; d++) {
-                        in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
-                        jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
-                    }
-                    if (!in && !jov) continue;
-                    for (int i = ib; i < ie; i++) {
-                        const double dx = p[0] - t.x[i], dy = p[1] - t.y[i], dz = p[2] - t.z[i];
-                        const double hm = r[i] > hj ? r[i] : hj;
-                        if (dx * dx + dy * dy + dz * dz < hm * hm) {
-                            if (FILL) jout[j0 + nj] = j;
-                            nj++;
-                            break;
-                        }
-                    }
-                }
-            } else {
-                open = true;
-            }
-        } else {
-            if (t.mo.mm[a] == 0.) continue;
-            const double c[3] = {t.mo.mx[a], t.mo.my[a], t.mo.mz[a]};
-            double s = 0.;
-            for (int d = 0; d < 3; d++) {
-                double e = 0.;
-                if (c[d] < glo[d])
-                    e = glo[d] - c[d];
-                else if (c[d] > ghi[d])
-                    e = c[d] - ghi[d];
-                s += e * e;
-            }
-            const double L = t.nd.len[a];
-            if (s > L * L * inv_t2) {
+    const int64_t j0 = FILL ? oj[tid] : 0;
+    const int64_t s0 = (FILL && MODE == 2) ? os[tid] : 0;
+
+    // ancestors of f, root first
+    int32_t anc[kMaxLevel + 1];
+    int na = 0;
+    for (int a = t.nd.parent[f]; a >= 0; a = t.nd.parent[a]) anc[na++] = a;
+    bool reach = true;
+    for (int q = na - 1; q >= 0 && reach; q--) {
+        const int a = anc[q];
+        const Visit v = visit<MODE>(t, C, a, olo, ohi, glo, ghi, inv_t2);
+        if (v == kOpen) continue;
+        reach = false;
+        if (v == kSpj && t.nd.begin[a] == t.nd.begin[f]) {
+            if (FILL) sout[s0 + ns] = a;
+            ns++;
+        }
+    }
+    if (reach) {
+        int32_t st[kStack];
+        int sp = 0;
+        st[sp++] = f;
+        while (sp > 0) {
+            const int a = st[--sp];
+            const Visit v = visit<MODE>(t, C, a, olo, ohi, glo, ghi, inv_t2);
+            if (v == kSkip) continue;
+            if (v == kSpj) {
                 if (FILL) sout[s0 + ns] = a;
                 ns++;
-            } else if (c0 < 0) {
-                for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
+                continue;
+            }
+            if (v == kOpen) {
+                const int c0 = t.nd.child[a], nc = t.nd.nchild[a];
+                if (sp + nc > kStack) {   // cannot happen for depth <= 21; stop cleanly
+                    atomicOr(err, 1);
+                    break;
+                }
+                for (int cc = c0 + nc - 1; cc >= c0; cc--) st[sp++] = cc;
+                continue;
+            }
+            // leaf: its particles, filtered by exact membership for the short lists
+            for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
+                if (MODE == 2) {
                     if (FILL) jout[j0 + nj] = j;
                     nj++;
+                    continue;
                 }
-            } else {
-                open = true;
+                const double p[3] = {t.x[j], t.y[j], t.z[j]};
+                const double hj = MODE == 1 ? r[j] : 0.;
+                bool in = true, jov = MODE == 1;
+                for (int d = 0; d < 3; d++) {
+                    in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
+                    if (MODE == 1) jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
+                }
+                if (!in && !jov) continue;
+                for (int i = ib; i < ie; i++) {
+                    const double dx = p[0] - t.x[i], dy = p[1] - t.y[i], dz = p[2] - t.z[i];
+                    const double rr = MODE == 1 ? (r[i] > hj ? r[i] : hj) : r[i];
+                    if (dx * dx + dy * dy + dz * dz < rr * rr) {
+                        if (FILL) jout[j0 + nj] = j;
+                        nj++;
+                        break;
+                    }
+                }
             }
-        }
-        if (open) {
-            if (sp + nc > kStack) {   // cannot happen for depth <= 21; stop cleanly
-                atomicOr(err, 1);
-                break;
-            }
-            for (int cc = c0 + nc - 1; cc >= c0; cc--) st[sp++] = cc;
         }
     }
     if (!FILL) {
-        cj[g] = nj;
-        if (MODE == 2) cs[g] = ns;
+        cj[tid] = nj;
+        if (MODE == 2) cs[tid] = ns;
     }
 }
 
-__global__ void offsets_kernel(int ng, const int64_t *__restrict__ scan, const int64_t *__restrict__ cnt,
+// Frontier nodes: the nodes of level L and the leaves above it.
+__global__ __launch_bounds__(kBlock) void front_mark_kernel(int nn, NodeArrays nd, int L, int32_t *__restrict__ head)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nn) return;
+    const int lv = nd.level[a];
+    if (lv == L || (lv < L && nd.child[a] < 0)) head[nd.begin[a]] = a;
+}
+
+__global__ __launch_bounds__(kBlock) void front_write_kernel(int n, const int32_t *__restrict__ head,
+                                                             const int64_t *__restrict__ pos,
+                                                             int32_t *__restrict__ front)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && head[i] >= 0) front[pos[i]] = head[i];
+}
+
+// Walk offsets from the scanned (walk, frontier) counts: off[g] = scan[g nf].
+__global__ void offsets_kernel(int ng, int nf, const int64_t *__restrict__ scan, const int64_t *__restrict__ cnt,
                                int32_t *__restrict__ off, int64_t *__restrict__ total)
 {
     const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g < ng) off[g] = (int32_t)scan[g];
+    if (g < ng) off[g] = (int32_t)scan[(int64_t)g * nf];
     if (g == ng - 1) {
-        const int64_t t = scan[g] + cnt[g];
+        const int64_t last = (int64_t)ng * nf - 1;
+        const int64_t t = scan[last] + cnt[last];
         *total = t;
         off[ng] = (int32_t)t;
     }
@@ -579,8 +643,9 @@ struct sphk_dtree {
     std::vector<int> level_start;   // node index of the first node of each level
     int n_node = 0;
     DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
-    DArr<int32_t> head, group, i_off;
-    int n_walk = 0;
+    DArr<int32_t> head, group, i_off, front;
+    int n_walk = 0, n_front = 0;
+    DArr<double> wbox;
     DArr<int64_t> i64a, i64b, i64c, i64d;
     DArr<int64_t> total;
     DArr<int> err;
@@ -598,7 +663,9 @@ struct sphk_dtree {
                         &sz, &se, &olo, &ohi})
             a->release();
         key.release();
-        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off}) a->release();
+        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &front})
+            a->release();
+        wbox.release();
         for (auto *a : {&i64a, &i64b, &i64c, &i64d, &total}) a->release();
         err.release();
         scratch.release();
@@ -649,28 +716,36 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
 {
     DLists &L = t->lists[MODE];
     L.built = false;
-    const int ng = t->n_walk;
-    SPHK_TRY(t->i64a.reserve(ng, false, t->st));   // EPJ counts
-    SPHK_TRY(t->i64b.reserve(ng, false, t->st));   // SPJ counts
-    SPHK_TRY(t->i64c.reserve(ng, false, t->st));   // scans
+    const int ng = t->n_walk, nf = t->n_front;
+    const int64_t nt = (int64_t)ng * nf;
+    SPHK_TRY(t->i64a.reserve(nt, false, t->st));   // EPJ counts per (walk, frontier node)
+    SPHK_TRY(t->i64b.reserve(nt, false, t->st));   // SPJ counts
+    SPHK_TRY(t->i64c.reserve(nt, false, t->st));   // EPJ offsets
+    SPHK_TRY(t->i64d.reserve(nt, false, t->st));   // SPJ offsets
     SPHK_TRY(t->total.reserve(2, false, t->st));
     SPHK_TRY(t->err.reserve(1, false, t->st));
     SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
     if (MODE == 2) SPHK_TRY(L.s_off.reserve(ng + 1, false, t->st));
+    if (MODE != 2) {
+        SPHK_TRY(t->wbox.reserve(6 * (size_t)ng, false, t->st));
+        hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p,
+                           t->y.p, t->z.p, r, t->wbox.p);
+    }
     SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
     TreeView v{t->nodes(), t->moments(), t->olo.p, t->ohi.p, t->x.p, t->y.p, t->z.p, t->group.p, t->i_off.p};
-    const dim3 lg((ng + 63) / 64), lb(64);
-    hipLaunchKernelGGL((list_kernel<MODE, false>), lg, lb, 0, t->st, ng, v, r, inv_t2, t->i64a.p, t->i64b.p,
-                       nullptr, nullptr, nullptr, nullptr, t->err.p);
+    const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
+    hipLaunchKernelGGL((list_kernel<MODE, false>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
+                       t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
     SPHK_TRY(hipGetLastError());
     int64_t tot[2] = {0, 0};
-    if (int rc = scan64(t, t->i64a.p, t->i64c.p, ng)) return rc;
-    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i64c.p, t->i64a.p,
+    if (nt > std::numeric_limits<int>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many walk subtrees");
+    if (int rc = scan64(t, t->i64a.p, t->i64c.p, (int)nt)) return rc;
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64c.p, t->i64a.p,
                        L.j_off.p, t->total.p);
     if (int rc = read1(t->st, t->total.p, &tot[0])) return rc;
     if (MODE == 2) {
-        if (int rc = scan64(t, t->i64b.p, t->i64c.p, ng)) return rc;
-        hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i64c.p, t->i64b.p,
+        if (int rc = scan64(t, t->i64b.p, t->i64d.p, (int)nt)) return rc;
+        hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64d.p, t->i64b.p,
                            L.s_off.p, t->total.p + 1);
         if (int rc = read1(t->st, t->total.p + 1, &tot[1])) return rc;
     }
@@ -678,8 +753,8 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
         return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
     SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
     if (MODE == 2) SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
-    hipLaunchKernelGGL((list_kernel<MODE, true>), lg, lb, 0, t->st, ng, v, r, inv_t2, nullptr, nullptr,
-                       L.j_off.p, MODE == 2 ? L.s_off.p : nullptr, L.j_idx.p,
+    hipLaunchKernelGGL((list_kernel<MODE, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
+                       nullptr, nullptr, t->i64c.p, MODE == 2 ? t->i64d.p : nullptr, L.j_idx.p,
                        MODE == 2 ? L.s_idx.p : nullptr, t->err.p);
     SPHK_TRY(hipGetLastError());
     int e = 0;
@@ -831,6 +906,30 @@ int sphk_dtree_build(int n, const double *x, const double *y, const double *z, c
     if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
         return fail(set_error(SPHK_E_DEVICE, "dtree: build failed"));
     t->n_walk = ng;
+
+    // frontier for the split list walks: the first level with >= 64 nodes
+    // (or the deepest), plus the leaves above it
+    int L = (int)t->level_start.size() - 2;
+    for (int q = 1; q + 1 < (int)t->level_start.size(); q++)
+        if (t->level_start[q + 1] - t->level_start[q] >= 64) {
+            L = q;
+            break;
+        }
+    hipLaunchKernelGGL(fill_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, -1);
+    hipLaunchKernelGGL(front_mark_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, st, (int)nn, t->nodes(), L,
+                       t->head.p);
+    hipLaunchKernelGGL(head_flag_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64a.p);
+    if (int rc = scan64(t, t->i64a.p, t->i64c.p, n)) return fail(rc);
+    if (hipMemcpyAsync(&last[0], t->i64c.p + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&last[1], t->i64a.p + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: frontier count read failed"));
+    t->n_front = (int)(last[0] + last[1]);
+    if (t->front.reserve(t->n_front, false, st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    hipLaunchKernelGGL(front_write_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64c.p, t->front.p);
+    if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: frontier failed"));
     *out = t;
     return SPHK_OK;
 }
