@@ -174,6 +174,19 @@ def main() -> int:
     acc_mixed = [sysm.s[c].clone() for c in ("ax", "ay", "az")]
     udot_mixed = sysm.s["udot"].clone()
     res = {}
+    # the same step captured in a HIP graph and replayed
+    graph = sysm.capture_force(MIXED)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g_ms = 0.0
+    for _ in range(args.steps):
+        restore()
+        ev0.record()
+        graph.replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        g_ms += ev0.elapsed_time(ev1)
+    res["ms_per_step_hipgraph"] = round(g_ms / args.steps, 4)
+    del graph
     if not args.no_fp64:
         log("timing fp64")
         ms_fp64, last_fp64, kern_fp64 = run(FP64, 1, max(3, args.steps // 4))

@@ -173,6 +173,7 @@ class Context:
     # --- timing (HIP events recorded around each launch on its stream) ---
     def timing(self, enable: bool = True):
         check(lib().sphk_timing(self._h, int(bool(enable))), "sphk_timing")
+        self._timing = bool(enable)
 
     def kernel_time(self, which: int) -> tuple[float, int]:
         ms = C.c_double()

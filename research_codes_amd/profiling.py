@@ -28,7 +28,8 @@ COPY_KERNEL = "copyBuffer"
 
 
 def _short(name: str) -> str:
-    return name.split("(")[0].replace("void ", "")
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0]
 
 
 def kernel_trace(stats_csv: str, trace_csv: str, last: int = 5) -> list[dict]:
@@ -61,12 +62,18 @@ def _per_dispatch(path: str) -> "OrderedDict[int, tuple[str, float]]":
 def traffic(fetch_csv: str, write_csv: str, kernel: str, last: int = 5) -> dict:
     """HBM bytes per launch of `kernel` (substring of its name), calibrated.
 
-    The calibration copies are the copy dispatches whose counter reads above a
-    quarter of COPY_BYTES; the scale is COPY_BYTES over their mean reading."""
+    The calibration copies are the first run of consecutive copy dispatches
+    of the program (bench.py issues them before anything else); the scale is
+    COPY_BYTES over their mean reading."""
     res = {}
     for label, path in (("fetch", fetch_csv), ("write", write_csv)):
         d = _per_dispatch(path)
-        copies = [v for name, v in d.values() if COPY_KERNEL in name and v * 1024 > COPY_BYTES / 4]
+        copies = []
+        for name, v in d.values():
+            if COPY_KERNEL in name and v * 1024 > COPY_BYTES / 4:
+                copies.append(v)
+            elif copies:
+                break
         if not copies:
             raise ValueError(f"{path}: no calibration copy dispatch found")
         scale = COPY_BYTES / (statistics.fmean(copies) * 1024)

@@ -363,6 +363,27 @@ class SPHSystem:
         check(lib().sphk_hydro_writeback_dev(self.ctx.handle, self.n, self._ss, self.phys, ho,
                                              self._st()), "sphk_hydro_writeback_dev")
 
+    def capture_force(self, precision=None):
+        """Record one force evaluation (density, EOS, gravity, hydro and their
+        write-backs, 10 kernel launches) on the current lists into a HIP graph
+        and return it: `graph.replay()` re-runs the step on the state arrays
+        in place.  The lists and state tensors must stay the ones captured;
+        kernel timing is switched off while capturing."""
+        timing = getattr(self.ctx, "_timing", False)
+        self.ctx.timing(False)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.force(precision)            # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.force(precision)
+        torch.cuda.synchronize()
+        self.ctx.timing(timing)
+        return g
+
     def force(self, precision=None):
         """One force evaluation on resident lists (no list rebuild, no retry)."""
         self.density(precision=precision)

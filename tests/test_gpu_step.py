@@ -263,3 +263,32 @@ def test_ensemble_batched_equals_per_run(prec):
     for k in whole:
         got = np.concatenate([q[k] for q in parts])
         assert np.array_equal(whole[k], got), k
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_captured_force_step_replays_bitwise(prec):
+    """The force step captured in a HIP graph gives the eager step's results
+    on every replay, with eager work and synchronisations between replays."""
+    p = ic.head_on(3000, eta=1.18)
+    with Context(0) as c:
+        s = SPHSystem(p, c, eta=1.18, eps=0.01, precision=prec, n_group=16)
+        s.initialize()
+        s.rs.copy_(1.1 * s.s["ksr"])
+        s.build_lists()
+        snap = {k: v.clone() for k, v in s.s.items()}
+
+        def restore():
+            for k, v in snap.items():
+                s.s[k].copy_(v)
+
+        restore()
+        s.force(prec)
+        ref = {k: s.s[k].clone() for k in ("dens", "ksr", "ax", "ay", "az", "udot", "pot", "vsmx")}
+        g = s.capture_force(prec)
+        for rep in range(4):
+            restore()
+            _ = (s.s["x"] * 2.0).sum().item()     # eager work between replays
+            g.replay()
+            torch.cuda.synchronize()
+            for k, v in ref.items():
+                assert torch.equal(s.s[k], v), (rep, k)
