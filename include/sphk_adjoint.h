@@ -1,0 +1,53 @@
+/*
+ * sphk_adjoint.h -- vector-Jacobian product of the gravity functor (C-ABI).
+ *
+ * Forward (sphk_gravity_dev with uniform softening, hdr_gravity.hpp:295-338
+ * with every eps2 equal): for particle i, over its EPJ list (particles j)
+ * and SPJ list (node monopoles: mass M, centre of mass X),
+ *     a_i   = - sum m_j d (r^2 + e2)^(-3/2),   d = x_i - x_j,
+ *     pot_i = - sum m_j (r^2 + e2)^(-1/2)      (self term included).
+ * Given cotangents abar (3 x N) and pbar (N), sphk_gravity_vjp_dev adds
+ *     d/dx_i:  sum_j m_j ( -T_ij abar_i + pbar_i w3 d )
+ *     d/dx_j:        m_j (  T_ij abar_i - pbar_i w3 d )
+ * with T = w3 I - 3 w5 d d^T, w = (r^2 + e2)^(-1/2), into gx/gy/gz for the
+ * particles of EPJ entries and into nx/ny/nz for the nodes of SPJ entries
+ * (FP64 throughout; atomic accumulation, so the summation order is not
+ * fixed).  sphk_dtree_spread_dev then carries the node cotangents to the
+ * particles through X_n = sum_{k in n} m_k x_k / M_n (d X_n / d x_k =
+ * m_k / M_n, every node containing k).  The lists and the tree are held
+ * fixed (piecewise-constant in x), as in any tree code's derivative.
+ * The masses are not differentiated.
+ *
+ * This operation has no counterpart in the reference; it serves callers that
+ * differentiate collision outcomes with respect to initial conditions.
+ */
+#ifndef SPHK_ADJOINT_H
+#define SPHK_ADJOINT_H
+
+#include "sphk.h"
+#include "sphk_dtree.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Accumulates (does not clear) gx, gy, gz (particles, N) and nx, ny, nz
+ * (nodes; may be NULL when there is no SPJ list).  pj and pi give particle
+ * m, x, y, z (eps2 unused: e2 is the uniform softening squared); ps the node
+ * monopoles m, x, y, z.  Device pointers; stream as in sphk.h. */
+int sphk_gravity_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
+                         const int32_t *j_idx, sphk_grav_src pj, const int32_t *s_off,
+                         const int32_t *s_idx, sphk_grav_src ps, sphk_grav_src pi, double e2,
+                         const double *abx, const double *aby, const double *abz, const double *pbar,
+                         double *gx, double *gy, double *gz, double *nx, double *ny, double *nz,
+                         void *stream);
+
+/* gx[k] += m_k * sum over the nodes n containing k of nx[n] / M_n (same for
+ * y, z); m = the particle masses the tree's monopoles were built from. */
+int sphk_dtree_spread_dev(const sphk_dtree *t, const double *m, const double *nx, const double *ny,
+                          const double *nz, double *gx, double *gy, double *gz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

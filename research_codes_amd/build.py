@@ -34,7 +34,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", f"-I{INCLUDE}", f"-I{CSRC}",
              "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-Wall"]
 
-HIP_SOURCES = ["sphk_kernels.hip", "sphk_runtime.hip", "sphk_step.hip", "sphk_dtree.hip"]
+HIP_SOURCES = ["sphk_kernels.hip", "sphk_runtime.hip", "sphk_step.hip", "sphk_dtree.hip", "sphk_adjoint.hip"]
 CXX_SOURCES = ["sphk_tree.cpp", "sphk_fdps.cpp"]
 
 

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "sphk.h"
+#include "sphk_adjoint.h"
 #include "sphk_dtree.h"
 #include "sphk_internal.hpp"
 
@@ -1021,6 +1022,69 @@ int sphk_dtree_nodes(const sphk_dtree *t, const int32_t **begin, const int32_t *
     if (len) *len = t->len.p;
     return SPHK_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// cotangent per unit mass of each node's centre of mass, summed over the node
+// and its ancestors (at most 21 levels up)
+__global__ __launch_bounds__(kBlock) void spread_node_kernel(int nn, NodeArrays nd, const double *__restrict__ M,
+                                                             const double *__restrict__ nx,
+                                                             const double *__restrict__ ny,
+                                                             const double *__restrict__ nz, double *__restrict__ cx,
+                                                             double *__restrict__ cy, double *__restrict__ cz)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nn) return;
+    double s[3] = {0., 0., 0.};
+    for (int b = a; b >= 0; b = nd.parent[b])
+        if (M[b] > 0.) {
+            s[0] += nx[b] / M[b];
+            s[1] += ny[b] / M[b];
+            s[2] += nz[b] / M[b];
+        }
+    cx[a] = s[0];
+    cy[a] = s[1];
+    cz[a] = s[2];
+}
+
+__global__ __launch_bounds__(kBlock) void spread_leaf_kernel(int nn, NodeArrays nd, const double *__restrict__ m,
+                                                             const double *__restrict__ cx,
+                                                             const double *__restrict__ cy,
+                                                             const double *__restrict__ cz, double *gx, double *gy,
+                                                             double *gz)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nn || nd.child[a] >= 0) return;
+    for (int k = nd.begin[a]; k < nd.end[a]; k++) {
+        gx[k] += m[k] * cx[a];
+        gy[k] += m[k] * cy[a];
+        gz[k] += m[k] * cz[a];
+    }
+}
+
+}  // namespace
+
+extern "C" int sphk_dtree_spread_dev(const sphk_dtree *t, const double *m, const double *nx, const double *ny,
+                                     const double *nz, double *gx, double *gy, double *gz)
+{
+    if (!t || !m || !nx || !ny || !nz || !gx || !gy || !gz) return set_error(SPHK_E_ARG, "dtree spread: null argument");
+    sphk_dtree *tt = const_cast<sphk_dtree *>(t);
+    const int nn = t->n_node;
+    // three per-node sums, kept in the SPJ scratch of the tree
+    SPHK_TRY(tt->sx.reserve(nn, false, t->st));
+    SPHK_TRY(tt->sy.reserve(nn, false, t->st));
+    SPHK_TRY(tt->sz.reserve(nn, false, t->st));
+    hipLaunchKernelGGL(spread_node_kernel, dim3(grid(nn)), dim3(kBlock), 0, t->st, nn, tt->nodes(), t->mm.p, nx, ny,
+                       nz, tt->sx.p, tt->sy.p, tt->sz.p);
+    hipLaunchKernelGGL(spread_leaf_kernel, dim3(grid(nn)), dim3(kBlock), 0, t->st, nn, tt->nodes(), m, tt->sx.p,
+                       tt->sy.p, tt->sz.p, gx, gy, gz);
+    SPHK_TRY(hipGetLastError());
+    return SPHK_OK;
+}
+
+extern "C" {
 
 int sphk_copy_dev(void *dst, const void *src, size_t bytes, void *stream)
 {
