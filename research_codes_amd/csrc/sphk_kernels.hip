@@ -244,7 +244,7 @@ __device__ __forceinline__ void dens_reduce_grad(double (&acc)[5], double *red, 
 // T = ceil(C / (64 - n)) steps (C entries over n particles).  If an H_k of
 // the iteration exceeds R_i the chunk continues with full sweeps.
 constexpr int kDensCap = 256;
-constexpr int kDensComp = 1472;
+constexpr int kDensComp = 1728;
 constexpr int kDensPad = kDensCap;   // inert record after the list, target of idle steps
 static_assert(kDensCap <= 256, "compacted positions are stored as bytes");
 
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kWave) void density_kernel(
 {
     using R = real_t<MIXED>;
     constexpr int kWords = kDensCap / kTile;
-    constexpr int kRed = (MIXED ? 5 * kWave * 4 : 5 * kWave * 8) / 8;   // doubles: 5 partials per lane
+    constexpr int kRed = MIXED ? 2 * kWave : 5 * kWave;   // doubles (mixed reduces two at a time)
     __shared__ double4 sp[kDensCap + 1];     // x, y, z, m
     __shared__ R4<R> sv[kDensCap + 1];       // vx, vy, vz, id
     __shared__ uint8_t cidx[kDensComp];
@@ -366,7 +366,6 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     __shared__ int s_own[kWave];
     __shared__ uint16_t s_ci[kWave], s_coff[kWave];
     int *s_cnt = reinterpret_cast<int *>(red);   // membership counts, before any reduction
-    R *redr = reinterpret_cast<R *>(red);        // grad-h partials of the compacted sweeps
 
     if ((int)blockIdx.x >= n_walk) return;
     const int w = xcd_walk(blockIdx.x, n_walk);
@@ -571,28 +570,44 @@ __global__ __launch_bounds__(kWave) void density_kernel(
         const R vx = (R)I.vx, vy = (R)I.vy, vz = (R)I.vz;
         double acc[5] = {0., 0., 0., 0., 0.};
         if (compact) {
+            // pair terms summed in R over at most 16 entries, then folded
+            // into FP64 (as the full sweep folds each 64-record tile)
             Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+            double part[5] = {0., 0., 0., 0., 0.};
+            auto fold = [&]() {
+                part[0] += (double)a.g;
+                part[1] += (double)a.dv;
+                part[2] += (double)a.rx;
+                part[3] += (double)a.ry;
+                part[4] += (double)a.rz;
+                a = Pass2<R>{R(0), R(0), R(0), R(0), R(0)};
+            };
             int st = 0, e = e0;
             for (; st + 2 <= nsteps; st += 2, e += 2 * stride) {
                 const int k0 = e < n_e ? myidx[e] : kDensPad;
                 const int k1 = e + stride < n_e ? myidx[e + stride] : kDensPad;
                 dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
                 dens_pass2_pair<R>(sp[k1], sv[k1], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                if ((st & 14) == 14) fold();
             }
             if (st < nsteps) {
                 const int k0 = e < n_e ? myidx[e] : kDensPad;
                 dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
             }
-            const R part[5] = {a.g, a.dv, a.rx, a.ry, a.rz};
-            __syncthreads();
-            for (int q = 0; q < 5; q++) redr[q * kWave + lane] = part[q];
-            __syncthreads();
-            if (owner)
-                for (int q = 0; q < 5; q++) {
-                    double s = (double)redr[q * kWave + lane];
-                    for (int t = 1; t < stride; t++) s += (double)redr[q * kWave + lane + t];
-                    acc[q] = s;
-                }
+            fold();
+            // per-i sums over the i's lanes in FP64, two quantities per round
+            for (int q0 = 0; q0 < 5; q0 += 2) {
+                const int nq = q0 + 1 < 5 ? 2 : 1;
+                __syncthreads();
+                for (int q = 0; q < nq; q++) red[q * kWave + lane] = part[q0 + q];
+                __syncthreads();
+                if (owner)
+                    for (int q = 0; q < nq; q++) {
+                        double s = red[q * kWave + lane];
+                        for (int t = 1; t < stride; t++) s += red[q * kWave + lane + t];
+                        acc[q0 + q] = s;
+                    }
+            }
         } else {
             for (int t = 0; t < ntile; t++) {
                 const double4 *tp = sp + t * kTile;
