@@ -16,6 +16,7 @@ import pytest
 import torch
 
 from research_codes_amd import device as D
+from research_codes_amd import ic
 from tests._cases import polytrope_case, scaled_err
 
 pytestmark = pytest.mark.gpu
@@ -123,3 +124,52 @@ def test_gravity_uniform_softening(ctx, prec):
         e = scaled_err(out[k].cpu().numpy(), ref[k], floor_frac=1e-1 if k in ("ax", "ay", "az") else 1e-3)
         print(f"gravity-uniform[{prec}] {k}: {e:.3e}")
         assert e < tol, k
+
+
+def _density_vs_oracle(ctx, p, i_off, j_off, j_idx, eta, hmax, prec, label):
+    import oracle
+    ref = oracle.density(i_off, j_off, j_idx, p, p, eta=eta, ksrh=1.936492, hmax=hmax)
+    ctx.set_params(eta=eta, hmax=hmax)
+    w = D.Walks(i_off, j_off, j_idx)
+    src = _dev(p, D.L.DENS_SRC)
+    out = D.empty_dens_out(len(p["m"]))
+    ctx.density(w, src, src, out, prec)
+    torch.cuda.synchronize()
+    got = {k: v.cpu().numpy() for k, v in out.items()}
+    n_itr = int(ref["itr"].sum())
+    print(f"[{label}] itr set for {n_itr} of {len(p['m'])}")
+    assert np.array_equal(got["np"], ref["np"]), label
+    assert np.array_equal(got["itr"], ref["itr"]), label
+    assert scaled_err(got["ksr"], ref["ksr"]) < 1e-12, label
+    assert scaled_err(got["dens"], ref["dens"]) < 1e-12, label
+    tol = RTOL_FP64 if prec == D.FP64 else RTOL_MIXED
+    for k in ("divv", "rotv", "grdh"):
+        assert scaled_err(got[k], ref[k], floor_frac=1e-3 if prec == D.FP64 else 1e-1) < tol, (label, k)
+    return n_itr
+
+
+@pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
+def test_density_compaction_paths(ctx, prec):
+    """The compacted density sweeps against the oracle where they switch
+    paths: a search radius equal to H (many H_k exceed it: full-sweep
+    fallback, itr set), a wide radius (compacted lists overflow one chunk),
+    and a walk of 125 particles whose list fits in LDS (i-chunks > 64)."""
+    from research_codes_amd.tree import HostTree, sort_particles
+    p = ic.polytrope(1.5, 3000, 1.0, 1.0, seed=42, eta=1.6)
+    p, _ = sort_particles(p)
+    t = HostTree(p["x"], p["y"], p["z"], p["m"], None, 8, 32)
+    for label, fac in (("tight", 1.0), ("wide", 2.5)):
+        q = dict(p)
+        q["rs"] = fac * p["ksr"]
+        n_itr = _density_vs_oracle(ctx, q, t.i_off, *t.gather(q["rs"]), 1.6, 10.0, prec, label)
+        if label == "tight":
+            assert n_itr > 0
+    t.close()
+    # one walk of a 5x5x5 lattice, every particle in the list, H ~ 2 spacings
+    g = np.stack(np.meshgrid(*[np.arange(5.0)] * 3, indexing="ij"), -1).reshape(-1, 3) * 0.1
+    n = len(g)
+    lat = {"x": g[:, 0].copy(), "y": g[:, 1].copy(), "z": g[:, 2].copy(), "m": np.full(n, 1e-3),
+           "vx": 0.1 * g[:, 1], "vy": -0.2 * g[:, 0], "vz": np.zeros(n), "id": np.arange(n, dtype=np.int32),
+           "ksr": np.full(n, 0.21), "rs": np.full(n, 0.3)}
+    _density_vs_oracle(ctx, lat, np.array([0, n], np.int32), np.array([0, n], np.int32),
+                       np.arange(n, dtype=np.int32), 1.6, 10.0, prec, "lattice125")
