@@ -500,7 +500,7 @@ __device__ __forceinline__ Visit visit(const TreeView &t, size_t C, int a, const
 // Concatenated in frontier order, the threads' outputs are the host's list.
 // MODE 0 gather, 1 symmetry, 2 long; with FILL entries are written from the
 // scanned offsets, else counted.
-template <int MODE, bool FILL>
+template <int MODE, bool FILL, bool LEAVES = false>
 __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, const int32_t *__restrict__ front,
                                                   const double *__restrict__ wb, const double *__restrict__ r,
                                                   double inv_t2, int64_t *__restrict__ cj,
@@ -569,6 +569,11 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
                 for (int cc = c0 + nc - 1; cc >= c0; cc--) st[sp++] = cc;
                 continue;
             }
+            if (LEAVES) {   // candidate leaf; its particles are filtered by member_kernel
+                if (FILL) jout[j0 + nj] = a;
+                nj++;
+                continue;
+            }
             // leaf: its particles, filtered by exact membership for the short lists
             for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
                 if (MODE == 2) {
@@ -600,6 +605,111 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
         cj[tid] = nj;
         if (MODE == 2) cs[tid] = ns;
     }
+}
+
+// Exact membership of the particles of each walk's candidate leaves, one
+// wave per walk: the walk's i-particles sit in LDS, lane l tests candidate
+// c0 + l of the walk's leaves laid end to end (the host's order), and a
+// ballot orders the accepted ones.  MODE 0: |x_j - x_i| < rs_i for some i;
+// MODE 1: r_ij < max(H_i, H_j) for some i (after the same box pre-tests as
+// the host's leaf loop).  With FILL the accepted j are written from off[g],
+// else counted into cnt[g].
+constexpr int kMemberCap = 256;   // i-particles of a walk held in LDS
+
+template <int MODE, bool FILL>
+__global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const int32_t *__restrict__ lo,
+                                                    const int32_t *__restrict__ li, const double *__restrict__ wb,
+                                                    const double *__restrict__ r, int64_t *__restrict__ cnt,
+                                                    const int32_t *__restrict__ off, int32_t *__restrict__ jout)
+{
+#pragma clang fp contract(off)
+    __shared__ double si[4][kMemberCap];
+    __shared__ int32_t lbeg[64], lpre[65];
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int lane = threadIdx.x;
+    const size_t C = t.mo.cap;
+    const int ib = t.i_off[g], ie = t.i_off[g + 1];
+    const int ni = ie - ib;
+    const bool in_lds = ni <= kMemberCap;
+    if (in_lds)
+        for (int q = lane; q < ni; q += 64) {
+            si[0][q] = t.x[ib + q];
+            si[1][q] = t.y[ib + q];
+            si[2][q] = t.z[ib + q];
+            si[3][q] = r[ib + q];
+        }
+    const int gn = t.group[g];
+    double olo[3], ohi[3], glo[3], ghi[3];
+    for (int d = 0; d < 3; d++) {
+        olo[d] = wb[d * ng + g];
+        ohi[d] = wb[(3 + d) * ng + g];
+        glo[d] = t.mo.lo[d * C + gn];
+        ghi[d] = t.mo.hi[d * C + gn];
+    }
+    int64_t count = 0;
+    const int64_t base = FILL ? (int64_t)off[g] : 0;
+    const int lb = lo[g], le = lo[g + 1];
+    for (int l0 = lb; l0 < le; l0 += 64) {
+        const int nl = min(64, le - l0);
+        __syncthreads();
+        int sz = 0;
+        if (lane < nl) {
+            const int a = li[l0 + lane];
+            lbeg[lane] = t.nd.begin[a];
+            sz = t.nd.end[a] - t.nd.begin[a];
+        }
+        // inclusive scan of the leaf sizes
+        int incl = sz;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        lpre[lane + 1] = incl;
+        if (lane == 0) lpre[0] = 0;
+        __syncthreads();
+        const int total = lpre[nl];
+        for (int c0 = 0; c0 < total; c0 += 64) {
+            const int c = c0 + lane;
+            bool pass = false;
+            int j = -1;
+            if (c < total) {
+                int lo_ = 0, hi_ = nl - 1;   // leaf q with lpre[q] <= c < lpre[q+1]
+                while (lo_ < hi_) {
+                    const int mid = (lo_ + hi_ + 1) >> 1;
+                    if (lpre[mid] <= c)
+                        lo_ = mid;
+                    else
+                        hi_ = mid - 1;
+                }
+                j = lbeg[lo_] + (c - lpre[lo_]);
+                const double p[3] = {t.x[j], t.y[j], t.z[j]};
+                const double hj = MODE == 1 ? r[j] : 0.;
+                bool in = true, jov = MODE == 1;
+                for (int d = 0; d < 3; d++) {
+                    in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
+                    if (MODE == 1) jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
+                }
+                if (in || jov)
+                    for (int q = 0; q < ni; q++) {
+                        const double xi = in_lds ? si[0][q] : t.x[ib + q];
+                        const double yi = in_lds ? si[1][q] : t.y[ib + q];
+                        const double zi = in_lds ? si[2][q] : t.z[ib + q];
+                        const double ri = in_lds ? si[3][q] : r[ib + q];
+                        const double dx = p[0] - xi, dy = p[1] - yi, dz = p[2] - zi;
+                        const double rr = MODE == 1 ? (ri > hj ? ri : hj) : ri;
+                        if (dx * dx + dy * dy + dz * dz < rr * rr) {
+                            pass = true;
+                            break;
+                        }
+                    }
+            }
+            const uint64_t bal = __ballot(pass);
+            if (FILL && pass) jout[base + count + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+            count += __popcll(bal);
+        }
+    }
+    if (!FILL && lane == 0) cnt[g] = count;
 }
 
 // Frontier nodes: the nodes of level L and the leaves above it.
@@ -644,7 +754,7 @@ struct sphk_dtree {
     std::vector<int> level_start;   // node index of the first node of each level
     int n_node = 0;
     DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
-    DArr<int32_t> head, group, i_off, front;
+    DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx;
     int n_walk = 0, n_front = 0;
     DArr<double> wbox;
     DArr<int64_t> i64a, i64b, i64c, i64d;
@@ -664,7 +774,8 @@ struct sphk_dtree {
                         &sz, &se, &olo, &ohi})
             a->release();
         key.release();
-        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &front})
+        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &front, &leaf_off,
+                        &leaf_idx})
             a->release();
         wbox.release();
         for (auto *a : {&i64a, &i64b, &i64c, &i64d, &total}) a->release();
@@ -713,8 +824,12 @@ int read1(hipStream_t st, const T *dev, T *host)
 int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
 template <int MODE>
+int build_list_two_stage(sphk_dtree *t, const double *r);
+
+template <int MODE>
 int build_list(sphk_dtree *t, const double *r, double inv_t2)
 {
+    if (MODE != 2) return build_list_two_stage<MODE>(t, r);
     DLists &L = t->lists[MODE];
     L.built = false;
     const int ng = t->n_walk, nf = t->n_front;
@@ -763,6 +878,67 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
     if (e) return set_error(SPHK_E_STATE, "dtree: traversal stack exceeded");
     L.n_j = tot[0];
     L.n_s = MODE == 2 ? tot[1] : 0;
+    L.built = true;
+    return SPHK_OK;
+}
+
+// Short-range lists in two stages: the split tree walk lists each walk's
+// candidate leaves (count, scan, fill), then member_kernel filters their
+// particles (count, scan, fill).  Same output as the one-stage walk.
+template <int MODE>
+int build_list_two_stage(sphk_dtree *t, const double *r)
+{
+    DLists &L = t->lists[MODE];
+    L.built = false;
+    const int ng = t->n_walk, nf = t->n_front;
+    const int64_t nt = (int64_t)ng * nf;
+    if (nt > std::numeric_limits<int>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many walk subtrees");
+    SPHK_TRY(t->i64a.reserve(nt, false, t->st));   // leaf counts per (walk, frontier node)
+    SPHK_TRY(t->i64c.reserve(nt, false, t->st));   // their scan
+    SPHK_TRY(t->total.reserve(2, false, t->st));
+    SPHK_TRY(t->err.reserve(1, false, t->st));
+    SPHK_TRY(t->leaf_off.reserve(ng + 1, false, t->st));
+    SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
+    SPHK_TRY(t->wbox.reserve(6 * (size_t)ng, false, t->st));
+    hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p, t->y.p,
+                       t->z.p, r, t->wbox.p);
+    SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
+    TreeView v{t->nodes(), t->moments(), t->olo.p, t->ohi.p, t->x.p, t->y.p, t->z.p, t->group.p, t->i_off.p};
+    const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
+    hipLaunchKernelGGL((list_kernel<MODE, false, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
+                       t->i64a.p, nullptr, nullptr, nullptr, nullptr, nullptr, t->err.p);
+    SPHK_TRY(hipGetLastError());
+    if (int rc = scan64(t, t->i64a.p, t->i64c.p, (int)nt)) return rc;
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64c.p, t->i64a.p,
+                       t->leaf_off.p, t->total.p);
+    int64_t n_leaves = 0;
+    if (int rc = read1(t->st, t->total.p, &n_leaves)) return rc;
+    if (n_leaves > std::numeric_limits<int32_t>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many leaves");
+    SPHK_TRY(t->leaf_idx.reserve(std::max<int64_t>(n_leaves, 1), false, t->st));
+    hipLaunchKernelGGL((list_kernel<MODE, true, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
+                       nullptr, nullptr, t->i64c.p, nullptr, t->leaf_idx.p, nullptr, t->err.p);
+    SPHK_TRY(hipGetLastError());
+    // stage 2: exact membership, one wave per walk
+    SPHK_TRY(t->i64b.reserve(ng, false, t->st));
+    SPHK_TRY(t->i64d.reserve(ng, false, t->st));
+    hipLaunchKernelGGL((member_kernel<MODE, false>), dim3(ng), dim3(64), 0, t->st, ng, v, t->leaf_off.p,
+                       t->leaf_idx.p, t->wbox.p, r, t->i64b.p, nullptr, nullptr);
+    SPHK_TRY(hipGetLastError());
+    if (int rc = scan64(t, t->i64b.p, t->i64d.p, ng)) return rc;
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, 1, t->i64d.p, t->i64b.p,
+                       L.j_off.p, t->total.p);
+    int64_t tot = 0;
+    if (int rc = read1(t->st, t->total.p, &tot)) return rc;
+    if (tot > std::numeric_limits<int32_t>::max()) return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
+    SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot, 1), false, t->st));
+    hipLaunchKernelGGL((member_kernel<MODE, true>), dim3(ng), dim3(64), 0, t->st, ng, v, t->leaf_off.p, t->leaf_idx.p,
+                       t->wbox.p, r, nullptr, L.j_off.p, L.j_idx.p);
+    SPHK_TRY(hipGetLastError());
+    int e = 0;
+    if (int rc = read1(t->st, t->err.p, &e)) return rc;
+    if (e) return set_error(SPHK_E_STATE, "dtree: traversal stack exceeded");
+    L.n_j = tot;
+    L.n_s = 0;
     L.built = true;
     return SPHK_OK;
 }
