@@ -637,9 +637,23 @@ __global__ __launch_bounds__(kWave) void density_kernel(
 // ---------------------------------------------------------------------------
 // Hydro: scalar calcHydro, sph_yamauchi/run/hdr_hydro.hpp:320-421.
 // ---------------------------------------------------------------------------
+// Position record of a hydro j-entry: FP64 absolute in the FP64 path; in the
+// mixed path FP32 offsets from the walk's first particle, formed in FP64 (as
+// for gravity: 16-byte records, no FP64 arithmetic per pair).
+template <typename R>
+struct HPos;
+template <>
+struct HPos<double> {
+    using rec = double4;
+};
+template <>
+struct HPos<float> {
+    using rec = float4;
+};
+
 template <typename R>
 struct HydroI {
-    double px, py, pz;
+    double px, py, pz;   // FP64 path: absolute; mixed path: offsets (in the FP32 range)
     R vx, vy, vz, hi, hi4, pp, pc, bs, dn, cs, al, au, eg, et;
     int32_t id;
 };
@@ -650,15 +664,15 @@ struct HydroAcc {
 };
 
 template <typename R>
-__device__ __forceinline__ void hydro_pair(const double4 &p, const R4<R> &A, const R4<R> &B,
+__device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const R4<R> &A, const R4<R> &B,
                                            const R4<R> &Cc, R eta_j, int32_t jid,
                                            const HydroI<R> &I, HydroAcc<R> &a)
 {
     // p = {x, y, z, m}; A = {vx, vy, vz, hinv}; B = {pres, presc, bswt, dens};
     // Cc = {vsnd, alph, alphu, thrm}
-    const R dpx = (R)(I.px - p.x);
-    const R dpy = (R)(I.py - p.y);
-    const R dpz = (R)(I.pz - p.z);
+    const R dpx = (R)I.px - (R)p.x;
+    const R dpy = (R)I.py - (R)p.y;
+    const R dpz = (R)I.pz - (R)p.z;
     const R dvx = I.vx - A.a;
     const R dvy = I.vy - A.b;
     const R dvz = I.vz - A.c;
@@ -708,10 +722,11 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
     using R = real_t<MIXED>;
     // tile records and the slice-reduction buffer share one LDS block (the
     // reduction runs after the last tile)
-    constexpr int kTileBytes = kTile * (sizeof(double4) + 3 * sizeof(R4<R>) + sizeof(R) + 4);
+    using PR = typename HPos<R>::rec;
+    constexpr int kTileBytes = kTile * (sizeof(PR) + 3 * sizeof(R4<R>) + sizeof(R) + 4);
     constexpr int kRedBytes = 9 * kWave * sizeof(double);
     __shared__ __attribute__((aligned(32))) char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
-    double4 *sp = reinterpret_cast<double4 *>(smem);
+    PR *sp = reinterpret_cast<PR *>(smem);
     R4<R> *sa = reinterpret_cast<R4<R> *>(sp + kTile);
     R4<R> *sb = sa + kTile;
     R4<R> *sc = sb + kTile;
@@ -725,13 +740,15 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
     const int ib = i_off[w], ie = i_off[w + 1];
     const int jb = j_off[w], je = j_off[w + 1];
 
+    // mixed: positions as offsets from the walk's first particle
+    const double cx = MIXED ? pi.x[ib] : 0., cy = MIXED ? pi.y[ib] : 0., cz = MIXED ? pi.z[ib] : 0.;
     for (int c0 = ib; c0 < ie; c0 += kWave) {
         const Map M(min(kWave, ie - c0), lane);
         const int i = c0 + M.il;
         HydroI<R> I;
-        I.px = pi.x[i];
-        I.py = pi.y[i];
-        I.pz = pi.z[i];
+        I.px = pi.x[i] - cx;
+        I.py = pi.y[i] - cy;
+        I.pz = pi.z[i] - cz;
         I.vx = (R)pi.vx[i];
         I.vy = (R)pi.vy[i];
         I.vz = (R)pi.vz[i];
@@ -756,14 +773,14 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
             __syncthreads();
             if (lane < nt) {
                 const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                sp[lane] = make_double4(pj.x[j], pj.y[j], pj.z[j], pj.m[j]);
+                sp[lane] = PR{(R)(pj.x[j] - cx), (R)(pj.y[j] - cy), (R)(pj.z[j] - cz), (R)pj.m[j]};
                 sa[lane] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)pj.hinv[j]};
                 sb[lane] = R4<R>{(R)pj.pres[j], (R)pj.presc[j], (R)pj.bswt[j], (R)pj.dens[j]};
                 sc[lane] = R4<R>{(R)pj.vsnd[j], (R)pj.alph[j], (R)pj.alphu[j], (R)pj.thrm[j]};
                 se[lane] = (R)pj.eta[j];
                 sid[lane] = pj.id[j];
             } else {
-                sp[lane] = make_double4(kFar, kFar, kFar, 0.);
+                sp[lane] = PR{(R)kFar, (R)kFar, (R)kFar, R(0)};
                 sa[lane] = R4<R>{R(0), R(0), R(0), R(0)};
                 sb[lane] = R4<R>{R(0), R(0), R(0), R(1)};
                 sc[lane] = R4<R>{R(-1e20), R(0), R(0), R(0)};
@@ -775,7 +792,7 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
             int k = M.slice;
             for (; k + M.S < kTile; k += 2 * M.S) {
                 const int k1 = k + M.S;
-                const double4 p0 = sp[k], p1 = sp[k1];
+                const PR p0 = sp[k], p1 = sp[k1];
                 const R4<R> a0 = sa[k], a1 = sa[k1], b0 = sb[k], b1 = sb[k1], c0_ = sc[k], c1 = sc[k1];
                 const R e0 = se[k], e1 = se[k1];
                 const int32_t d0 = sid[k], d1 = sid[k1];
