@@ -289,6 +289,9 @@ def ensemble_main(args) -> int:
         return (time.perf_counter() - t) / steps * 1e3
 
     ms_batched = timed(lambda: s.force(MIXED), args.steps)
+    graph = s.capture_force(MIXED)
+    ms_graph = timed(graph.replay, args.steps)
+    del graph
     # the same walks launched run by run (one launch per functor per run)
     full = dict(s.lists)
     per_run = []
@@ -323,6 +326,7 @@ def ensemble_main(args) -> int:
            "data": "synthetic (seeded Lane-Emden polytropes)",
            "config": {"workload": "ensemble of 64 parabolic collisions, N_run log-uniform in [1e4, 2e5], "
                                   "q ~ U(0.3,1), b ~ U(0,1)", "N": s.n, "runs": len(counts), "n_group": g},
+           "ms_per_step_hipgraph": round(ms_graph, 4),
            "ms_per_step_run_by_run": round(ms_seq, 4), "speedup_batched": round(ms_seq / ms_batched, 3),
            "entries": entries}
     print(json.dumps(out), flush=True)

@@ -1,0 +1,45 @@
+"""Sensitivity of a collision outcome to the impactor's initial conditions,
+by central finite differences of the full SPH simulation.
+
+The loss is the smoothed unbound mass sum m sigma(k e) after a fixed number
+of KDK steps with a fixed timestep (a power-of-two timestep choice would
+make the loss piecewise in the parameters).  The gravity functor also has a
+hand-written adjoint (research_codes_amd.adjoint); the SPH functors do not,
+so the full-system gradient here is the finite-difference one.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import ic
+from .device import MIXED, Context
+from .step import SPHSystem
+
+
+def head_on_loss(dx0: float = 0.0, dv0: float = 0.0, n_each: int = 100_000, steps: int = 200,
+                 dt: float = 1e-3, k: float = 20.0, eta: float = 1.18, eps: float = 0.01,
+                 precision: int = MIXED) -> float:
+    """Smoothed unbound mass after `steps` KDK steps of the head-on collision
+    of two n=1.5 stars, the impactor (star 1) shifted by dx0 and its velocity
+    changed by dv0 along the collision axis."""
+    p = ic.head_on(n_each, eta=eta)
+    imp = p["istar"] == 1
+    p["x"] = np.where(imp, p["x"] + dx0, p["x"])
+    p["vx"] = np.where(imp, p["vx"] + dv0, p["vx"])
+    with Context(0) as c:
+        s = SPHSystem(p, c, eta=eta, eps=eps, precision=precision, n_group=32)
+        s.initialize()
+        for _ in range(steps):
+            s.kdk(dt=dt)
+        return s.unbound_mass(k)["unbound_smooth"]
+
+
+def fd_gradient(fn, params, h: float = 1e-4) -> np.ndarray:
+    """Central differences of fn(*params) with step h in each parameter."""
+    params = np.asarray(params, np.float64)
+    g = np.zeros_like(params)
+    for i in range(len(params)):
+        e = np.zeros_like(params)
+        e[i] = h
+        g[i] = (fn(*(params + e)) - fn(*(params - e))) / (2 * h)
+    return g

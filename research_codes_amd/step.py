@@ -441,11 +441,13 @@ class SPHSystem:
         self.hydro()
         self.update_hmax()
 
-    def kdk(self):
+    def kdk(self, dt: float | None = None):
         """One loopSimulation step (hdr_hgas.hpp:1499-1539) with a fixed list
-        rebuild per step.  Returns the diagnostics before the step."""
+        rebuild per step.  dt overrides the power-of-two timestep rule (a
+        fixed schedule, e.g. for finite-difference sensitivities).  Returns
+        the diagnostics before the step."""
         d = self.diagnostics()
-        self.dt = self.next_dt(d["dt_min"])
+        self.dt = self.next_dt(d["dt_min"]) if dt is None else float(dt)
         dt = self.dt
         check(lib().sphk_predict_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_predict_dev")
