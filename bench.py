@@ -66,7 +66,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --n-group for all kernels instead of tuning 8/16/32/64 per kernel")
     ap.add_argument("--groups", default=None,
-                    help="walk group sizes dens,hydro,grav (e.g. 16,16,32); implies --no-autotune")
+                    help="walk group sizes dens,hydro,grav[,density list capacity] (e.g. 32,32,32,256); "
+                         "implies --no-autotune")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run 3 plain 256 MiB device copies (counter scale reference)")
     ap.add_argument("--workload", choices=("collision", "ensemble", "drift"), default="collision",
@@ -111,6 +112,8 @@ def main() -> int:
         args.autotune = False
     sysm = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=n_group,
                      precision=MIXED)
+    if args.groups and len(args.groups.split(",")) > 3:
+        ctx.set_params(list_cap=int(args.groups.split(",")[3]))
     t0 = time.perf_counter()
     rounds = sysm.relax_h(3)
     sysm.eos_balsara()
@@ -233,6 +236,7 @@ def main() -> int:
                    "walks": {k: W[k].n_walk for k in W}},
         "autotune_ms": None if tune is None else {k: {str(g): round(t, 4) for g, t in v.items()}
                                                   for k, v in tune[1].items()},
+        "dens_list_cap": sysm.ctx.params["list_cap"] or 256,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "entries": entries, "flop_per_step": flops,

@@ -76,6 +76,11 @@ typedef struct {
      * softening.  The symmetrised pair term then needs one rsqrt instead of
      * two; results are those of the general formula.  0 (default): general. */
     int uniform_softening;
+    /* Density functor: capacity (records) of a walk's j-list kept in LDS for
+     * its four sweeps, 192 or 256; longer lists stream through 64-record
+     * tiles.  192 needs less LDS (more waves per CU), 256 keeps more walks
+     * resident.  0 (default): 256. */
+    int list_cap;
 } sphk_params;
 
 /* ---- inputs / outputs (all FP64 unless stated) ------------------------ */

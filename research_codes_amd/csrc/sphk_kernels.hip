@@ -243,9 +243,8 @@ __device__ __forceinline__ void dens_reduce_grad(double (&acc)[5], double *red, 
 // together; the lanes are shared out so that every taken i needs at most
 // T = ceil(C / (64 - n)) steps (C entries over n particles).  If an H_k of
 // the iteration exceeds R_i the chunk continues with full sweeps.
-constexpr int kDensCap = 256;
+constexpr int kDensCap = 256;        // largest resident capacity (instantiated: 192, 256)
 constexpr int kDensComp = 1728;
-constexpr int kDensPad = kDensCap;   // inert record after the list, target of idle steps
 static_assert(kDensCap <= 256, "compacted positions are stored as bytes");
 
 template <bool MIXED>
@@ -349,17 +348,19 @@ __device__ void density_streaming(int lane, const Map &M, int i, int jb, int je,
     if (M.lead) dens_finish(out, i, acc, dens, np, hs, I.rs);
 }
 
-template <bool MIXED>
+template <bool MIXED, int CAP>
 __global__ __launch_bounds__(kWave) void density_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_dens_src pi, sphk_dens_src pj, sphk_dens_out out,
     double eta_ksrh, double hmax)
 {
     using R = real_t<MIXED>;
-    constexpr int kWords = kDensCap / kTile;
+    static_assert(CAP % kTile == 0 && CAP <= kDensCap, "list capacity");
+    constexpr int kWords = CAP / kTile;
+    constexpr int kDensPad = CAP;   // inert record after the list, target of idle steps
     constexpr int kRed = MIXED ? 2 * kWave : 5 * kWave;   // doubles (mixed reduces two at a time)
-    __shared__ double4 sp[kDensCap + 1];     // x, y, z, m
-    __shared__ R4<R> sv[kDensCap + 1];       // vx, vy, vz, id
+    __shared__ double4 sp[CAP + 1];     // x, y, z, m
+    __shared__ R4<R> sv[CAP + 1];       // vx, vy, vz, id
     __shared__ uint8_t cidx[kDensComp];
     __shared__ double red[kRed];
     __shared__ double shs[kWave];
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(kWave) void density_kernel(
     const int nj = je - jb;
     const int ntile = (nj + kTile - 1) / kTile;
 
-    if (nj > kDensCap) {
+    if (nj > CAP) {
         for (int c0 = ib; c0 < ie; c0 += kWave) {
             const Map M(min(kWave, ie - c0), lane);
             density_streaming<MIXED>(lane, M, c0 + M.il, jb, je, j_idx, pi, pj, out, eta_ksrh, hmax,
@@ -1065,15 +1066,24 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
 // ---------------------------------------------------------------------------
 hipError_t launch_density(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_dens_src &pi, const sphk_dens_src &pj,
-                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed)
+                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap)
 {
     if (n_walk <= 0) return hipSuccess;
-    if (mixed)
-        hipLaunchKernelGGL(density_kernel<true>, dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,
-                           j_off, j_idx, pi, pj, out, eta_ksrh, hmax);
-    else
-        hipLaunchKernelGGL(density_kernel<false>, dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,
-                           j_off, j_idx, pi, pj, out, eta_ksrh, hmax);
+#define SPHK_LAUNCH_D(M, CAPV)                                                                          \
+    hipLaunchKernelGGL((density_kernel<M, CAPV>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, \
+                       j_idx, pi, pj, out, eta_ksrh, hmax)
+    if (list_cap == 192) {
+        if (mixed)
+            SPHK_LAUNCH_D(true, 192);
+        else
+            SPHK_LAUNCH_D(false, 192);
+    } else {
+        if (mixed)
+            SPHK_LAUNCH_D(true, 256);
+        else
+            SPHK_LAUNCH_D(false, 256);
+    }
+#undef SPHK_LAUNCH_D
     return hipGetLastError();
 }
 

@@ -228,11 +228,13 @@ class SPHSystem:
             self.mom = moms[0] if len(moms) == 1 else {k: torch.cat([m[k] for m in moms]) for k in moms[0]}
 
     def autotune_groups(self, candidates=(8, 16, 32), reps: int = 3, precision=None):
-        """Pick, per list kind, the walk group size with the fastest kernel.
+        """Pick, per list kind, the walk group size with the fastest kernel
+        (for density also the LDS list capacity, 192 or 256).
 
         Each candidate rebuilds that kind's lists and times `reps` launches
         of its functor (HIP events on the launch stream).  Leaves the best
-        lists in place; returns ({kind: size}, {kind: {size: ms}})."""
+        lists and settings in place; returns ({kind: size, "dens_list_cap":
+        cap}, {kind: {candidate: ms}})."""
         n = self.n
         tmp = {k: torch.empty(n, dtype=torch.float64, device=self.dev)
                for k in ("dens", "divv", "rotv", "grdh", "ksr")}
@@ -241,22 +243,37 @@ class SPHSystem:
         run = {"dens": lambda: self.density(out=tmp, precision=precision),
                "hydro": lambda: self.hydro(precision),
                "grav": lambda: self.gravity(precision)}
+        def timed(fn):
+            fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
         best, table = {}, {}
         for kind in ("dens", "hydro", "grav"):
             table[kind] = {}
             for g in candidates:
                 self.n_group[kind] = g
                 self.build_lists((kind,))
-                run[kind]()
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                for _ in range(reps):
-                    run[kind]()
-                e1.record()
-                torch.cuda.synchronize()
-                table[kind][g] = e0.elapsed_time(e1) / reps
-            best[kind] = min(table[kind], key=table[kind].get)
-            self.n_group[kind] = best[kind]
+                if kind == "dens":   # the density kernel's resident list capacity too
+                    for cap in (192, 256):
+                        self.ctx.set_params(list_cap=cap)
+                        table[kind][f"{g}/{cap}"] = timed(run[kind])
+                else:
+                    table[kind][str(g)] = timed(run[kind])
+            key = min(table[kind], key=table[kind].get)
+            if kind == "dens":
+                g, cap = (int(v) for v in key.split("/"))
+                self.ctx.set_params(list_cap=cap)
+                best["dens_list_cap"] = cap
+            else:
+                g = int(key)
+            best[kind] = g
+            self.n_group[kind] = g
             self.build_lists((kind,))
         return best, table
 

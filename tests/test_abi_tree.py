@@ -125,3 +125,4 @@ def test_fixed_groups_partition(star):
     assert np.array_equal(t.i_off[:-1], np.arange(0, len(p["m"]), 32))
     assert t.i_off[-1] == len(p["m"])
     t.close()
+

@@ -37,15 +37,19 @@ def case(request):
     return polytrope_case(N, seed=1000 + N, n_group=g)
 
 
+@pytest.mark.parametrize("cap", [256, 192], ids=["cap256", "cap192"])
 @pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
-def test_density(ctx, case, prec):
+def test_density(ctx, case, prec, cap):
     p = case["p"]
-    ctx.set_params(eta=case["eta"], hmax=case["hmax"])
+    ctx.set_params(eta=case["eta"], hmax=case["hmax"], list_cap=cap)
     w = D.Walks(case["i_off"], *case["dens_lists"])
     src = _dev(p, D.L.DENS_SRC)
     out = D.empty_dens_out(len(p["m"]))
-    ctx.density(w, src, src, out, prec)
-    torch.cuda.synchronize()
+    try:
+        ctx.density(w, src, src, out, prec)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_params(list_cap=0)
     ref = case["dens_out"]
     got = {k: v.cpu().numpy() for k, v in out.items()}
     assert np.array_equal(got["np"], ref["np"]), "neighbour counts differ"
@@ -173,3 +177,9 @@ def test_density_compaction_paths(ctx, prec):
            "ksr": np.full(n, 0.21), "rs": np.full(n, 0.3)}
     _density_vs_oracle(ctx, lat, np.array([0, n], np.int32), np.array([0, n], np.int32),
                        np.arange(n, dtype=np.int32), 1.6, 10.0, prec, "lattice125")
+
+
+def test_list_cap_parameter_is_validated(ctx):
+    with pytest.raises(D.L.SphkError):
+        ctx.set_params(list_cap=100)
+    ctx.set_params(list_cap=0)
