@@ -9,12 +9,15 @@ reduces them to the figures the benchmark reports:
 * HBM bytes per launch of one kernel from FETCH_SIZE / WRITE_SIZE (KiB per
   dispatch), each scaled by what the same counter reads for a plain
   device-to-device copy of known size (bench.py --copy-calibration runs three
-  256 MiB copies first).
+  256 MiB copies first);
+* any other counters of one `--pmc` pass per launch of one kernel (e.g. the
+  hardware-counted VALU flops beside the algorithmic ones).
 
 Usage:
   python -m research_codes_amd.profiling trace <kernel_stats.csv> <kernel_trace.csv> [last]
   python -m research_codes_amd.profiling traffic <fetch counter_collection.csv> \
       <write counter_collection.csv> <kernel substring> [last]
+  python -m research_codes_amd.profiling counters <counter_collection.csv> <kernel substring> [last]
 """
 from __future__ import annotations
 
@@ -86,6 +89,23 @@ def traffic(fetch_csv: str, write_csv: str, kernel: str, last: int = 5) -> dict:
     return res
 
 
+def counters(csv_path: str, kernel: str, last: int = 5) -> dict:
+    """Mean per launch of every counter in one counter-collection file, over
+    the last `last` dispatches of `kernel` (substring of its name)."""
+    per: "OrderedDict[int, dict]" = OrderedDict()
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            if kernel not in r["Kernel_Name"]:
+                continue
+            d = per.setdefault(int(r["Dispatch_Id"]), {})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    if not per:
+        raise ValueError(f"{csv_path}: no dispatch of {kernel}")
+    rows = list(per.values())[-last:]
+    names = sorted({k for r in rows for k in r})
+    return {k: statistics.fmean(r.get(k, 0.0) for r in rows) for k in names}
+
+
 def main(argv: list[str]) -> int:
     if len(argv) >= 3 and argv[0] == "trace":
         last = int(argv[3]) if len(argv) > 3 else 5
@@ -104,6 +124,12 @@ def main(argv: list[str]) -> int:
             print(f"{k}_scale={t[k]['scale']:.5f}")
             print(f"{k}_bytes_per_launch={t[k]['bytes_per_launch']:.0f}")
         print(f"traffic_bytes_per_launch={t['bytes_per_launch']:.0f}")
+        return 0
+    if len(argv) >= 3 and argv[0] == "counters":
+        last = int(argv[3]) if len(argv) > 3 else 5
+        print(f"kernel={argv[2]}")
+        for k, v in counters(argv[1], argv[2], last).items():
+            print(f"{k}={v:.6g}")
         return 0
     print(__doc__, file=sys.stderr)
     return 2
