@@ -70,10 +70,11 @@ def main() -> int:
                          "implies --no-autotune")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run 3 plain 256 MiB device copies (counter scale reference)")
-    ap.add_argument("--workload", choices=("collision", "ensemble", "drift"), default="collision",
+    ap.add_argument("--workload", choices=("collision", "ensemble", "drift", "adjoint"), default="collision",
                     help="collision: the N=1e6 off-axis step (default); ensemble: 64 independent "
                          "collisions (sum N ~ 4.3e6) in one launch per functor; drift: 2e5-particle "
-                         "head-on collision, --steps KDK steps in mixed and in FP64")
+                         "head-on collision, --steps KDK steps in mixed and in FP64; adjoint: gradient "
+                         "of a 2e5-particle collision's smoothed unbound mass after --steps steps")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU spatial decomposition "
@@ -83,6 +84,8 @@ def main() -> int:
         return ensemble_main(args)
     if args.workload == "drift":
         return drift_main(args)
+    if args.workload == "adjoint":
+        return adjoint_main(args)
 
     import torch
     from research_codes_amd import ic
@@ -380,6 +383,65 @@ def drift_main(args) -> int:
                res["mixed"]["drift"] / max(res["fp64"]["drift"], 1e-300),
            "ms_per_step_fp64": round(res["fp64"]["ms_per_step"], 3), "t_end": res["mixed"]["time"],
            "same_t_end": res["mixed"]["time"] == res["fp64"]["time"], "max_abs_pos_diff": dpos}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def adjoint_main(args) -> int:
+    """Head-on collision of two n=1.5 stars (M=R=1, 1e5 particles each)
+    starting in contact (separation 2), FP64, eps=0.01, theta=0.5: --steps
+    KDK steps with the power-of-two timestep, loss = smoothed unbound mass
+    sum m sigma(20 e).  Reverse-mode gradient with respect to the impactor's
+    offset and velocity change (checkpoints every 20 steps, the step lengths
+    held fixed), checked against central differences (h = 1e-4) of the same
+    HIP run on the same schedule."""
+    import torch
+    from research_codes_amd import ic
+    from research_codes_amd.device import FP64, Context
+    from research_codes_amd.sensitivity import fd_gradient, head_on_loss
+    from research_codes_amd.sph_adjoint import CollisionRun
+    from research_codes_amd.step import SPHSystem
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    sep, n_each = 2.0, 100_000
+    p = ic.head_on(n_each, eta=args.eta, separation=sep)
+    with Context(0) as ctx:
+        s = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=32, precision=FP64)
+        run = CollisionRun(s, None, checkpoint_every=20, k=20.0)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        t = time.perf_counter()
+        loss = run.forward(args.steps)
+        torch.cuda.synchronize()
+        fwd = time.perf_counter() - t
+        log(f"forward {args.steps} steps in {fwd:.2f}s, t_end={s.time:.4g}, loss={loss:.6e}")
+        t = time.perf_counter()
+        g = run.backward()
+        torch.cuda.synchronize()
+        bwd = time.perf_counter() - t
+        peak = torch.cuda.max_memory_allocated() / 2 ** 30
+        imp = (run.istar0 == 1).double()
+        grad = np.array([float((g["x"] * imp).sum()), float((g["vx"] * imp).sum())])
+        log(f"backward in {bwd:.2f}s, gradient {grad}, peak {peak:.1f} GiB")
+        dts, t_end = list(run.dts), s.time
+        del run, g, s
+    t = time.perf_counter()
+    fd = fd_gradient(lambda a, b: head_on_loss(a, b, n_each=n_each, eta=args.eta, precision=FP64,
+                                               theta=0.5, separation=sep, dts=dts), [0.0, 0.0], h=1e-4)
+    fd_s = time.perf_counter() - t
+    log(f"central differences {fd} in {fd_s:.1f}s")
+    out = {"metric": "reverse-mode gradient of a collision's smoothed unbound mass (2x1e5, FP64)",
+           "value": round(bwd / args.steps * 1e3, 3), "unit": "ms per step (backward)", "n_gpus": 1,
+           "steps": args.steps, "warmup": 0, "ms_per_step": round(fwd / args.steps * 1e3, 3),
+           "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded Lane-Emden polytropes)",
+           "config": {"workload": "head-on collision of two n=1.5 stars (M=R=1) in contact at t=0 "
+                                  "(separation 2), 2x1e5 particles, eps=0.01, theta=0.5, power-of-two dt",
+                      "N": 2 * n_each, "eta": args.eta, "checkpoint_every": 20, "k": 20.0},
+           "loss": loss, "t_end": t_end, "forward_s": round(fwd, 3), "backward_s": round(bwd, 3),
+           "peak_gpu_mem_gib": round(peak, 2), "grad_dx0_dv0": grad.tolist(), "fd_dx0_dv0": fd.tolist(),
+           "fd_h": 1e-4, "fd_s": round(fd_s, 2),
+           "rel_diff_vs_fd": (np.abs(grad - fd) / np.abs(fd).max()).tolist()}
     print(json.dumps(out), flush=True)
     return 0
 

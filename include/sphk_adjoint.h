@@ -5,10 +5,13 @@
  * with every eps2 equal): for particle i, over its EPJ list (particles j)
  * and SPJ list (node monopoles: mass M, centre of mass X),
  *     a_i   = - sum m_j d (r^2 + e2)^(-3/2),   d = x_i - x_j,
- *     pot_i = - sum m_j (r^2 + e2)^(-1/2)      (self term included).
- * Given cotangents abar (3 x N) and pbar (N), sphk_gravity_vjp_dev adds
- *     d/dx_i:  sum_j m_j ( -T_ij abar_i + pbar_i w3 d )
- *     d/dx_j:        m_j (  T_ij abar_i - pbar_i w3 d )
+ *     pot_i = - sum m_j (r^2 + e2)^(-1/2)      (self term included),
+ *     eta_i =   sum m_j (r^2 + e2)^(-3/2)      (the eta output, before the
+ *                                              write-back's H/Omega factor).
+ * Given cotangents abar (3 x N), pbar (N) and ebar (N), sphk_gravity_vjp_dev
+ * adds
+ *     d/dx_i:  sum_j m_j ( -T_ij abar_i + pbar_i w3 d - 3 ebar_i w5 d )
+ *     d/dx_j:        m_j (  T_ij abar_i - pbar_i w3 d + 3 ebar_i w5 d )
  * with T = w3 I - 3 w5 d d^T, w = (r^2 + e2)^(-1/2), into gx/gy/gz for the
  * particles of EPJ entries and into nx/ny/nz for the nodes of SPJ entries
  * (FP64 throughout; atomic accumulation, so the summation order is not
@@ -34,13 +37,14 @@ extern "C" {
 /* Accumulates (does not clear) gx, gy, gz (particles, N) and nx, ny, nz
  * (nodes; may be NULL when there is no SPJ list).  pj and pi give particle
  * m, x, y, z (eps2 unused: e2 is the uniform softening squared); ps the node
- * monopoles m, x, y, z.  Device pointers; stream as in sphk.h. */
+ * monopoles m, x, y, z.  ebar may be NULL (no eta cotangent).  Device
+ * pointers; stream as in sphk.h. */
 int sphk_gravity_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
                          const int32_t *j_idx, sphk_grav_src pj, const int32_t *s_off,
                          const int32_t *s_idx, sphk_grav_src ps, sphk_grav_src pi, double e2,
                          const double *abx, const double *aby, const double *abz, const double *pbar,
-                         double *gx, double *gy, double *gz, double *nx, double *ny, double *nz,
-                         void *stream);
+                         const double *ebar, double *gx, double *gy, double *gz, double *nx, double *ny,
+                         double *nz, void *stream);
 
 /* gx[k] += m_k * sum over the nodes n containing k of nx[n] / M_n (same for
  * y, z); m = the particle masses the tree's monopoles were built from. */

@@ -134,7 +134,7 @@ SIGNATURES.update({
     "sphk_unbound_mass_dev": (C.c_int, [_P, C.c_int, State, _P, C.c_int, C.c_double, _DP, _P]),
     # sphk_adjoint.h
     "sphk_gravity_vjp_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, GravSrc, _P, _P, GravSrc, GravSrc, C.c_double]
-                             + [_P] * 10 + [_P]),
+                             + [_P] * 11 + [_P]),
     "sphk_dtree_spread_dev": (C.c_int, [_P] + [_P] * 7),
 })
 

@@ -31,7 +31,9 @@ class TreeGravity:
     """a, pot of N particles with masses m and softening eps (uniform).
 
     Call with float64 CUDA position tensors (any order); returns
-    (ax, ay, az, pot) times G, differentiable with respect to the positions."""
+    (ax, ay, az, pot) times G, differentiable with respect to the positions.
+    `with_eta` also returns the functor's eta sum (sum m_j w^3, no G), the
+    input of the SPH grad-h gravity term."""
 
     def __init__(self, ctx: Context, m: torch.Tensor, eps: float, theta: float = 0.5, n_group: int = 32,
                  n_leaf: int = 8, precision: int = FP64, G: float = 1.0):
@@ -44,11 +46,17 @@ class TreeGravity:
         self.precision, self.G = precision, G
 
     def __call__(self, x, y, z):
+        return self.with_eta(x, y, z)[:4]
+
+    def with_eta(self, x, y, z):
         perm = morton_order_dev(x.detach(), y.detach(), z.detach())
+        if bool((perm == torch.arange(perm.numel(), device=perm.device)).all()):
+            out = _GravityFn.apply(x, y, z, perm, self)
+            return tuple(o * self.G for o in out[:4]) + (out[4],)
         inv = torch.empty_like(perm)
         inv[perm] = torch.arange(perm.numel(), device=perm.device)
         out = _GravityFn.apply(x[perm], y[perm], z[perm], perm, self)
-        return tuple(o[inv] * self.G for o in out)
+        return tuple(o[inv] * self.G for o in out[:4]) + (out[4][inv],)
 
     # forward on sorted particles; returns the outputs and what the backward
     # pass needs (the tree, its lists and monopoles)
@@ -61,9 +69,9 @@ class TreeGravity:
         src = dict(m=ms, x=x, y=y, z=z, eps2=e2)
         out = {k: torch.empty_like(ms) for k in L.GRAV_OUT}
         self.ctx.gravity(st.walks, src, src, out, ps=st.mom, precision=self.precision)
-        return (out["ax"], out["ay"], out["az"], out["pot"]), st
+        return (out["ax"], out["ay"], out["az"], out["pot"], out["eta"]), st
 
-    def _backward(self, st, x, y, z, gax, gay, gaz, gpot):
+    def _backward(self, st, x, y, z, gax, gay, gaz, gpot, geta):
         w, mom, t = st.walks, st.mom, st.tree
         n = x.numel()
         g = [torch.zeros(n, dtype=torch.float64, device=x.device) for _ in range(3)]
@@ -72,7 +80,7 @@ class TreeGravity:
                         eps2=st.e2.data_ptr())
         ps = L.GravSrc(m=mom["m"].data_ptr(), x=mom["x"].data_ptr(), y=mom["y"].data_ptr(),
                        z=mom["z"].data_ptr(), eps2=mom["eps2"].data_ptr())
-        cot = [c.contiguous() for c in (gax, gay, gaz, gpot)]
+        cot = [c.contiguous() for c in (gax, gay, gaz, gpot, geta)]
         check(lib().sphk_gravity_vjp_dev(self.ctx.handle, w.n_walk, w.i_off.data_ptr(), w.j_off.data_ptr(),
                                          w.j_idx.data_ptr(), src, w.s_off.data_ptr(), w.s_idx.data_ptr(), ps,
                                          src, self.e2, *(c.data_ptr() for c in cot), *(a.data_ptr() for a in g),
@@ -93,17 +101,17 @@ class _GravityFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, y, z, perm, op):
         x, y, z = x.contiguous(), y.contiguous(), z.contiguous()
-        (ax, ay, az, pot), st = op._forward(x, y, z, perm)
+        (ax, ay, az, pot, eta), st = op._forward(x, y, z, perm)
         ctx.op, ctx.st = op, st
         ctx.save_for_backward(x, y, z)
-        return ax, ay, az, pot
+        return ax, ay, az, pot, eta
 
     @staticmethod
-    def backward(ctx, gax, gay, gaz, gpot):
+    def backward(ctx, gax, gay, gaz, gpot, geta):
         x, y, z = ctx.saved_tensors
         z0 = torch.zeros_like(x)
         gx, gy, gz = ctx.op._backward(ctx.st, x, y, z,
-                                      *(z0 if c is None else c for c in (gax, gay, gaz, gpot)))
+                                      *(z0 if c is None else c for c in (gax, gay, gaz, gpot, geta)))
         return gx, gy, gz, None, None
 
 

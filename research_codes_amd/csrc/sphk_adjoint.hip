@@ -30,7 +30,8 @@ __device__ __forceinline__ int xcd_walk_adj(int b, int n_walk)
 
 __device__ void vjp_list(int lane, int S, int slice, int kb, int ke, const int32_t *__restrict__ idx,
                          const sphk_grav_src &src, double px, double py, double pz, double e2, double ax,
-                         double ay, double az, double pb, VRec *sp, double *jb, int32_t *sj, double (&gi)[3],
+                         double ay, double az, double pb, double eb, VRec *sp, double *jb, int32_t *sj,
+                         double (&gi)[3],
                          double *outx, double *outy, double *outz)
 {
     for (int t0 = kb; t0 < ke; t0 += kTile) {
@@ -57,9 +58,11 @@ __device__ void vjp_list(int lane, int S, int slice, int kb, int ke, const int32
             const double w5 = w3 * w * w;
             const double dot = dx * ax + dy * ay + dz * az;
             // T abar = w3 abar - 3 w5 d (d . abar); then subtract pbar w3 d
-            const double cx = r.m * (w3 * ax - 3. * w5 * dx * dot - pb * w3 * dx);
-            const double cy = r.m * (w3 * ay - 3. * w5 * dy * dot - pb * w3 * dy);
-            const double cz = r.m * (w3 * az - 3. * w5 * dz * dot - pb * w3 * dz);
+            // and add 3 ebar w5 d (the eta sum's gradient is -3 m w5 d)
+            const double f = 3. * w5 * (dot - eb) + pb * w3;
+            const double cx = r.m * (w3 * ax - f * dx);
+            const double cy = r.m * (w3 * ay - f * dy);
+            const double cz = r.m * (w3 * az - f * dz);
             gi[0] -= cx;
             gi[1] -= cy;
             gi[2] -= cz;
@@ -82,7 +85,8 @@ __global__ __launch_bounds__(kWave) void gravity_vjp_kernel(
     const int32_t *__restrict__ j_idx, sphk_grav_src pj, const int32_t *__restrict__ s_off,
     const int32_t *__restrict__ s_idx, sphk_grav_src ps, sphk_grav_src pi, double e2,
     const double *__restrict__ abx, const double *__restrict__ aby, const double *__restrict__ abz,
-    const double *__restrict__ pbar, double *gx, double *gy, double *gz, double *nx, double *ny, double *nz)
+    const double *__restrict__ pbar, const double *__restrict__ ebar, double *gx, double *gy, double *gz,
+    double *nx, double *ny, double *nz)
 {
     __shared__ VRec sp[kTile];
     __shared__ double jb[3 * kTile];
@@ -99,12 +103,13 @@ __global__ __launch_bounds__(kWave) void gravity_vjp_kernel(
         const int i = c0 + il;
         const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
         const double ax = abx[i], ay = aby[i], az = abz[i], pb = pbar[i];
+        const double eb = ebar ? ebar[i] : 0.;
         double gi[3] = {0., 0., 0.};
-        vjp_list(lane, S, slice, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, ax, ay, az, pb, sp, jb, sj,
-                 gi, gx, gy, gz);
+        vjp_list(lane, S, slice, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, ax, ay, az, pb, eb, sp, jb,
+                 sj, gi, gx, gy, gz);
         if (s_off)
-            vjp_list(lane, S, slice, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2, ax, ay, az, pb, sp, jb,
-                     sj, gi, nx, ny, nz);
+            vjp_list(lane, S, slice, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2, ax, ay, az, pb, eb, sp,
+                     jb, sj, gi, nx, ny, nz);
         __syncthreads();
         for (int q = 0; q < 3; q++) red[q * kWave + lane] = gi[q];
         __syncthreads();
@@ -126,8 +131,8 @@ extern "C" int sphk_gravity_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_
                                     const int32_t *j_idx, sphk_grav_src pj, const int32_t *s_off,
                                     const int32_t *s_idx, sphk_grav_src ps, sphk_grav_src pi, double e2,
                                     const double *abx, const double *aby, const double *abz,
-                                    const double *pbar, double *gx, double *gy, double *gz, double *nx,
-                                    double *ny, double *nz, void *stream)
+                                    const double *pbar, const double *ebar, double *gx, double *gy,
+                                    double *gz, double *nx, double *ny, double *nz, void *stream)
 {
     using sphk::set_error;
     if (!ctx) return set_error(SPHK_E_ARG, "gravity_vjp: null context");
@@ -141,7 +146,7 @@ extern "C" int sphk_gravity_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_
     if (!(e2 > 0.)) return set_error(SPHK_E_ARG, "gravity_vjp: softening must be positive");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(sphk::gravity_vjp_kernel, dim3(n_walk), dim3(sphk::kWave), 0, st, n_walk, i_off, j_off,
-                       j_idx, pj, s_off, s_idx, ps, pi, e2, abx, aby, abz, pbar, gx, gy, gz, nx, ny, nz);
+                       j_idx, pj, s_off, s_idx, ps, pi, e2, abx, aby, abz, pbar, ebar, gx, gy, gz, nx, ny, nz);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(SPHK_E_DEVICE, "gravity_vjp: %s", hipGetErrorString(e));
     return SPHK_OK;

@@ -165,11 +165,11 @@ def single_polytrope(N: int = 10_000, seed: int = 1000, eta: float = 1.18) -> di
     return polytrope(1.5, N, 1.0, 1.0, seed=seed, eta=eta)
 
 
-def head_on(N_each: int = 100_000, eta: float = 1.18) -> dict:
+def head_on(N_each: int = 100_000, eta: float = 1.18, separation: float = 3.0) -> dict:
     le = LaneEmden(1.5)
     a = polytrope(1.5, N_each, 1.0, 1.0, seed=2000, eta=eta, le=le)
     b = polytrope(1.5, N_each, 1.0, 1.0, seed=2001, eta=eta, le=le)
-    return collision(a, b, 3.0, 0.0)
+    return collision(a, b, separation, 0.0)
 
 
 def off_axis(N_total: int = 1_000_000, eta: float = 1.18) -> dict:

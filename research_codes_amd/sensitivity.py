@@ -3,9 +3,9 @@ by central finite differences of the full SPH simulation.
 
 The loss is the smoothed unbound mass sum m sigma(k e) after a fixed number
 of KDK steps with a fixed timestep (a power-of-two timestep choice would
-make the loss piecewise in the parameters).  The gravity functor also has a
-hand-written adjoint (research_codes_amd.adjoint); the SPH functors do not,
-so the full-system gradient here is the finite-difference one.
+make the loss piecewise in the parameters).  The reverse-mode gradient of
+the same run is research_codes_amd.sph_adjoint.head_on_gradient; these
+differences are its check.
 """
 from __future__ import annotations
 
@@ -18,19 +18,21 @@ from .step import SPHSystem
 
 def head_on_loss(dx0: float = 0.0, dv0: float = 0.0, n_each: int = 100_000, steps: int = 200,
                  dt: float = 1e-3, k: float = 20.0, eta: float = 1.18, eps: float = 0.01,
-                 precision: int = MIXED) -> float:
-    """Smoothed unbound mass after `steps` KDK steps of the head-on collision
-    of two n=1.5 stars, the impactor (star 1) shifted by dx0 and its velocity
+                 precision: int = MIXED, theta: float = 0.5, separation: float = 3.0,
+                 dts=None) -> float:
+    """Smoothed unbound mass after `steps` KDK steps of length dt (or the
+    schedule `dts`, one length per step) of the head-on collision of two
+    n=1.5 stars, the impactor (star 1) shifted by dx0 and its velocity
     changed by dv0 along the collision axis."""
-    p = ic.head_on(n_each, eta=eta)
+    p = ic.head_on(n_each, eta=eta, separation=separation)
     imp = p["istar"] == 1
     p["x"] = np.where(imp, p["x"] + dx0, p["x"])
     p["vx"] = np.where(imp, p["vx"] + dv0, p["vx"])
     with Context(0) as c:
-        s = SPHSystem(p, c, eta=eta, eps=eps, precision=precision, n_group=32)
+        s = SPHSystem(p, c, eta=eta, eps=eps, theta=theta, precision=precision, n_group=32)
         s.initialize()
-        for _ in range(steps):
-            s.kdk(dt=dt)
+        for h in (dts if dts is not None else [dt] * steps):
+            s.kdk(dt=h)
         return s.unbound_mass(k)["unbound_smooth"]
 
 

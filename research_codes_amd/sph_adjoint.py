@@ -1,0 +1,396 @@
+"""Reverse-mode gradient of a fixed-step SPH collision run.
+
+The forward run is the HIP path: SPHSystem.initialize, then KDK steps with a
+fixed dt, with the device state checkpointed every `checkpoint_every` steps.
+The backward pass re-runs each segment's HIP steps from its checkpoint
+(bitwise the same steps), recording per step the Morton reorder, the H_max
+and the density and hydro walk lists (SPHSystem.kdk(record=...)).  It then
+walks the segment backwards: each step is restated as a differentiable FP64
+function of its input state over exactly those lists and linearised at the
+recorded state, and the cotangent is pulled back with torch.autograd.grad.
+The restatement follows the functors and glue the HIP step runs:
+
+* predict / correct (hdr_hgas.hpp:646-665);
+* the density functor's three smoothing-length passes and the grad-h,
+  divergence and curl pass (hdr_density.hpp:254-363);
+* EOS and Balsara switch (hdr_hgas.hpp:318-344, hdr_sph.hpp:79-89);
+* tree gravity and its write-back (hdr_gravity.hpp:107-113, 295-338),
+  through the hand-written vector-Jacobian product (adjoint.TreeGravity);
+* the hydro functor (hdr_hydro.hpp:320-421), its EPI preparation
+  (hdr_hydro.hpp:50-60) and write-back with the acceleration sum and the
+  viscosity switches' rates (hdr_hydro.hpp:21-25, hdr_sph.hpp:95-97,
+  hdr_hgas.hpp:750-762).
+
+As in any tree code's derivative, the tree, the lists and the reorder are
+held fixed (piecewise constant in the state).  The SPH kernel vanishes with
+continuous derivatives at the list edges, so list changes do not make the
+outcome jump; gravity's multipole acceptance does (theta = 0 removes it).
+The smoothing-length cap H_max is held fixed (it binds only where H reaches
+it), and the timestep is fixed (the power-of-two rule makes the outcome
+piecewise constant in dt).  Gravity needs a uniform softening.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import ic
+from .adjoint import TreeGravity
+from .device import FP64, Context, Walks
+from .step import SPHSystem
+
+C0 = 21.0 / (2.0 * math.pi)      # Wendland C2 normalisation, hdr_kernel.hpp:226
+C1 = 4.0 * C0                    # hdr_kernel.hpp:227
+
+# per-particle state carried from step to step (SPHSystem.s field names)
+STATE = ("x", "y", "z", "vx", "vy", "vz", "uene", "alph", "alphu", "ax", "ay", "az", "udot", "adot",
+         "adotu", "ksr", "pot", "dens")
+
+
+# ----------------------------------------------------------------- kernels
+def _w0(q):
+    """W(q) = C0 (1-q)_+^4 (1+4q), hdr_kernel.hpp:246-250."""
+    t = torch.clamp(1.0 - q, min=0.0)
+    t2 = t * t
+    return C0 * t2 * t2 * (1.0 + 4.0 * q)
+
+
+def _w1(q):
+    """W'(q) = C1 [(1-q)_+^4 - (1-q)_+^3 (1+4q)], hdr_kernel.hpp:251-257."""
+    t = torch.clamp(1.0 - q, min=0.0)
+    t2 = t * t
+    return C1 * (t2 * t2 - t2 * t * (1.0 + 4.0 * q))
+
+
+def _sqrt0(s):
+    """sqrt with a zero derivative at s = 0 (the functors' 0 cases: self
+    pairs, zero curl, equal pressures)."""
+    pos = s > 0
+    return torch.where(pos, torch.sqrt(torch.where(pos, s, torch.ones_like(s))), torch.zeros_like(s))
+
+
+def _sum(n, idx, v):
+    return torch.zeros(n, dtype=v.dtype, device=v.device).index_add(0, idx, v)
+
+
+def expand_pairs(w: Walks) -> tuple[torch.Tensor, torch.Tensor]:
+    """(i, j) of every entry of every walk's list: i over the walk's
+    particles, j over its list (int64 device tensors, walk order)."""
+    io = w.i_off.long()
+    jo = w.j_off.long()
+    ni = io[1:] - io[:-1]
+    nj = jo[1:] - jo[:-1]
+    cnt = ni * nj
+    total = int(cnt.sum())
+    dev = io.device
+    if total == 0:
+        e = torch.zeros(0, dtype=torch.int64, device=dev)
+        return e, e
+    wk = torch.repeat_interleave(torch.arange(w.n_walk, device=dev), cnt)
+    start = torch.cumsum(cnt, 0) - cnt
+    loc = torch.arange(total, device=dev) - start[wk]
+    njw = nj[wk]
+    i = io[wk] + torch.div(loc, njw, rounding_mode="floor")
+    pos = jo[wk] + loc % njw
+    j = pos if w.j_idx is None else w.j_idx.long()[pos]
+    return i, j
+
+
+# ------------------------------------------------------------------ force
+def _density_t(P, m, pairs, hmax, eta, ksrh):
+    """The density functor on pairs (i, j): returns dens, ksr (= H_3), divv,
+    rotv and grdh (= Omega^-1), differentiable in the positions, velocities
+    and the starting H (P["ksr"])."""
+    i, j = pairs
+    n = m.numel()
+    dx, dy, dz = (P[c][i] - P[c][j] for c in ("x", "y", "z"))
+    r2 = dx * dx + dy * dy + dz * dz
+    r = _sqrt0(r2)
+    mj = m[j]
+    H = P["ksr"]
+    dens = None
+    for _ in range(3):                                  # hdr_density.hpp:264-302
+        hi = 1.0 / H
+        dens = _sum(n, i, mj * _w0(r * hi[i])) * (hi * hi * hi)
+        H = torch.clamp(eta * ksrh * (m / dens) ** (1.0 / 3.0), max=hmax)
+    hi = 1.0 / H
+    hi4 = hi * hi * hi * hi
+    self_ = i == j                                      # ids are unique: self = same index
+    ri = torch.where(self_, torch.zeros_like(r), 1.0 / torch.where(self_, torch.ones_like(r), r))
+    q = r2 * ri * hi[i]
+    kw0, kw1 = _w0(q), _w1(q)
+    grdh = -_sum(n, i, (3.0 * kw0 + q * kw1) * mj) * hi4
+    dw = mj * hi4[i] * kw1 * ri
+    dvx, dvy, dvz = (P[c][i] - P[c][j] for c in ("vx", "vy", "vz"))
+    divv = -_sum(n, i, (dvx * dx + dvy * dy + dvz * dz) * dw)
+    rx = _sum(n, i, (dvy * dz - dvz * dy) * dw)
+    ry = _sum(n, i, (dvz * dx - dvx * dz) * dw)
+    rz = _sum(n, i, (dvx * dy - dvy * dx) * dw)
+    omg = 1.0 / (1.0 + H / dens * grdh / 3.0)           # hdr_density.hpp:351-362
+    rot = _sqrt0(rx * rx + ry * ry + rz * rz)
+    return dens, H, divv / dens * omg, rot / dens * omg, omg
+
+
+def _density_radius(P, m, pairs, hmax, eta, ksrh):
+    """Largest H of the three passes per particle (no gradient): list
+    entries beyond it contribute nothing to any pass."""
+    with torch.no_grad():
+        i, j = pairs
+        n = m.numel()
+        r = torch.sqrt(sum((P[c][i] - P[c][j]) ** 2 for c in ("x", "y", "z")))
+        H = P["ksr"].detach()
+        R = H.clone()
+        for _ in range(3):
+            hi = 1.0 / H
+            dens = _sum(n, i, m[j] * _w0(r * hi[i])) * (hi * hi * hi)
+            H = torch.clamp(eta * ksrh * (m / dens) ** (1.0 / 3.0), max=hmax)
+            R = torch.maximum(R, H)
+        return r, R
+
+
+def _hydro_t(P, m, pairs, d, G):
+    """The hydro functor on non-self pairs (i, j) with r < max(H_i, H_j)."""
+    i, j = pairs
+    n = m.numel()
+    dx, dy, dz = (P[c][i] - P[c][j] for c in ("x", "y", "z"))
+    dvx, dvy, dvz = (P[c][i] - P[c][j] for c in ("vx", "vy", "vz"))
+    r = torch.sqrt(dx * dx + dy * dy + dz * dz)
+    ri = 1.0 / r
+    hi = d["hinv"]
+    hi4 = hi * hi * hi * hi
+    dwi = hi4[i] * _w1(r * hi[i])
+    dwj = hi4[j] * _w1(r * hi[j])
+    ka = (dwi + dwj) * m[j]
+    wij = (dx * dvx + dy * dvy + dz * dvz) * ri
+    w0 = torch.clamp(wij, max=0.0)
+    vs = d["vsnd"][i] + d["vsnd"][j] - 3.0 * w0
+    rhi = 1.0 / (d["dens"][i] + d["dens"][j])
+    av0 = (d["bswt_half"][i] + d["bswt_half"][j]) * (P["alph"][i] + P["alph"][j]) * vs * w0
+    pc = d["presc"]
+    ta = (pc[i] + pc[j] - 0.5 * av0 * rhi) * ka * ri
+    vsu = _sqrt0(torch.abs(d["pres"][i] - d["pres"][j]) * rhi * 2.0)
+    du = d["thrm"][i] - d["thrm"][j]
+    udot = _sum(n, i, ka * (pc[i] * wij - rhi * (0.25 * av0 * w0 - (P["alphu"][i] + P["alphu"][j]) * vsu * du)))
+    et = d["eta"]
+    dg = m[j] * ri * (et[i] * dwi + et[j] * dwj)
+    out = {"achx": -0.5 * _sum(n, i, ta * dx), "achy": -0.5 * _sum(n, i, ta * dy),
+           "achz": -0.5 * _sum(n, i, ta * dz), "udot": 0.5 * udot,
+           "diffu": 2.0 * _sum(n, i, rhi * du * ka * ri)}
+    for c, dc in (("x", dx), ("y", dy), ("z", dz)):
+        out["g1" + c] = G * 0.5 * _sum(n, i, dg * dc)
+    return out
+
+
+def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
+    """One force evaluation (density, EOS + Balsara, gravity + write-back,
+    hydro + write-back + rates) on the recorded lists, differentiable in
+    the positions, velocities, u, alpha, alpha_u and the starting H of P."""
+    ph = sysm.phys
+    ksrh = ph.ksrh
+    # density over the recorded gather lists, trimmed to the entries that
+    # can contribute (r below the largest H of the three passes)
+    i, j = expand_pairs(rec["dens"])
+    r, R = _density_radius(P, m, (i, j), rec["hmax"], sysm.eta, ksrh)
+    keep = r < R[i] * (1.0 + 1e-9)
+    dens, H, divv, rotv, omg = _density_t(P, m, (i[keep], j[keep]), rec["hmax"], sysm.eta, ksrh)
+    del i, j, r, R, keep
+    # EOS and Balsara switch
+    u = P["uene"]
+    pres = ph.ninv * dens * u
+    vsnd = _sqrt0((1.0 + ph.ninv) * pres / dens)
+    ad = torch.abs(divv)
+    bswt = ad / (ad + torch.abs(rotv) + 1e-4 * vsnd * ksrh / H)
+    # gravity (uniform softening) and its write-back
+    tg = TreeGravity(sysm.ctx, m.detach(), eps=sysm.eps, theta=sysm.theta, n_group=sysm.n_group["grav"],
+                     n_leaf=sysm.n_leaf, precision=FP64, G=ph.G)
+    g2x, g2y, g2z, pot, etag = tg.with_eta(P["x"], P["y"], P["z"])
+    eta = H * H / (ksrh * ksrh) * omg / (3.0 * dens) * etag
+    # hydro
+    d = {"hinv": 1.0 / H, "presc": pres * omg / (dens * dens), "bswt_half": 0.5 * bswt,
+         "thrm": torch.clamp(u, min=0.0), "vsnd": vsnd, "dens": dens, "pres": pres, "eta": eta}
+    i, j = expand_pairs(rec["hydro"])
+    with torch.no_grad():
+        r2 = sum((P[c][i] - P[c][j]) ** 2 for c in ("x", "y", "z"))
+        hmx = torch.maximum(H[i], H[j])
+        keep = (i != j) & (r2 < hmx * hmx)
+    h = _hydro_t(P, m, (i[keep], j[keep]), d, ph.G)
+    del i, j, r2, hmx, keep
+    # write-back: acceleration sum and the viscosity switches' rates
+    out = {"dens": dens, "ksr": H, "pot": pot, "udot": h["udot"]}
+    out["ax"] = h["achx"] + h["g1x"] + g2x
+    out["ay"] = h["achy"] + h["g1y"] + g2y
+    out["az"] = h["achz"] + h["g1z"] + g2z
+    alph, alphu = P["alph"], P["alphu"]
+    src = torch.clamp(-divv * (ph.alpha_max - alph), min=0.0)
+    tauinv = 0.25 * ksrh * vsnd / H
+    out["adot"] = -(alph - ph.alpha_min) * tauinv + src
+    lim = torch.clamp(ph.alphu_max - alphu, min=0.0)
+    on = lim > 0
+    srcu = torch.where(on, H / ksrh * torch.abs(h["diffu"])
+                       / torch.sqrt(torch.clamp(u, min=0.0) + ph.eps_u + (~on).double()) * lim,
+                       torch.zeros_like(lim))
+    out["adotu"] = -(alphu - ph.alphu_min) * tauinv + srcu
+    return out
+
+
+def init_t(S: dict, m, rec: dict, sysm: SPHSystem) -> dict:
+    """The initial force evaluation (SPHSystem.initialize) as a function of
+    the initial state."""
+    f = force_t(S, m, rec, sysm)
+    out = {k: S[k] for k in ("x", "y", "z", "vx", "vy", "vz", "uene", "alph", "alphu")}
+    out.update(f)
+    return out
+
+
+def step_t(S: dict, m, rec: dict, sysm: SPHSystem, dt: float) -> dict:
+    """One KDK step (SPHSystem.kdk with a fixed dt) as a function of the
+    state at its start; m is in that state's order."""
+    P = {}
+    for c, v in (("x", "vx"), ("y", "vy"), ("z", "vz")):
+        a = "a" + c
+        P[c] = S[c] + S[v] * dt + 0.5 * S[a] * dt * dt          # hdr_hgas.hpp:646-657
+        P[v] = S[v] + S[a] * dt
+        P[v + "2"] = S[v] + 0.5 * S[a] * dt
+    for k, kd in (("uene", "udot"), ("alph", "adot"), ("alphu", "adotu")):
+        P[k] = S[k] + S[kd] * dt
+        P[k + "2"] = S[k] + 0.5 * S[kd] * dt
+    P["ksr"] = S["ksr"]
+    perm = rec["perm"]
+    if perm is not None:
+        P = {k: v[perm] for k, v in P.items()}
+        m = m[perm]
+    f = force_t(P, m, rec, sysm)
+    out = {"x": P["x"], "y": P["y"], "z": P["z"]}
+    for v, a in (("vx", "ax"), ("vy", "ay"), ("vz", "az")):
+        out[v] = P[v + "2"] + 0.5 * f[a] * dt                   # hdr_hgas.hpp:659-665
+    for k, kd in (("uene", "udot"), ("alph", "adot"), ("alphu", "adotu")):
+        out[k] = P[k + "2"] + 0.5 * f[kd] * dt
+    out.update(f)
+    return out
+
+
+def smoothed_unbound_t(S: dict, m, k: float):
+    """sum m sigma(k e), e = |v - v_cg|^2 / 2 + pot + u, v_cg density-weighted
+    (analysis/unbound_mass/unbound_mass.py:10-24, analysis/common.py:71-90,
+    the smoothed form of sphk_unbound_mass_dev over all particles)."""
+    w = S["dens"]
+    W = w.sum()
+    c = [(w * S[v]).sum() / W for v in ("vx", "vy", "vz")]
+    e = 0.5 * sum((S[v] - cv) ** 2 for v, cv in zip(("vx", "vy", "vz"), c)) + S["pot"] + S["uene"]
+    return (m * torch.sigmoid(k * e)).sum()
+
+
+# ------------------------------------------------------------ checkpoints
+_EXTRA = ("id", "istar", "rs", "np_", "itr", "eps2")
+
+
+def _save(sysm: SPHSystem) -> dict:
+    ck = {"s": {k: v.clone() for k, v in sysm.s.items()},
+          "extra": {k: getattr(sysm, k).clone() for k in _EXTRA},
+          "hmax": sysm.hmax, "time": sysm.time, "dt": sysm.dt}
+    return ck
+
+
+def _restore(sysm: SPHSystem, ck: dict):
+    for k, v in ck["s"].items():
+        sysm.s[k].copy_(v)
+    for k, v in ck["extra"].items():
+        getattr(sysm, k).copy_(v)
+    sysm.hmax = ck["hmax"]
+    sysm.ctx.set_params(hmax=ck["hmax"])
+    sysm.time, sysm.dt = ck["time"], ck["dt"]
+
+
+def _state(sysm: SPHSystem) -> dict:
+    return {k: sysm.s[k].clone() for k in STATE}
+
+
+def _pull(out: dict, inp: dict, cot: dict) -> dict:
+    keys = [k for k in out if k in cot and out[k].requires_grad]
+    g = torch.autograd.grad([out[k] for k in keys], list(inp.values()), [cot[k] for k in keys],
+                            allow_unused=True)
+    return {k: (torch.zeros_like(v) if gk is None else gk) for (k, v), gk in zip(inp.items(), g)}
+
+
+class CollisionRun:
+    """An SPH run on the HIP path and its reverse-mode gradient.
+
+    `forward(steps)` runs SPHSystem.initialize and `steps` KDK steps,
+    checkpointing the state every `checkpoint_every` steps, and returns the
+    smoothed unbound mass of the final state.  The steps have length dt, or
+    with dt=None the power-of-two rule's lengths, which the gradient then
+    holds fixed (kept in `dts`).  `backward()` returns the gradient of the
+    loss with respect to the initial state (a dict of per-particle tensors
+    per field, in the order the system holds its particles at construction).
+    The system must be in FP64 precision with a uniform gravitational
+    softening."""
+
+    def __init__(self, sysm: SPHSystem, dt: float | None, checkpoint_every: int = 20, k: float = 20.0):
+        if sysm.eps is None:
+            raise ValueError("the gravity adjoint needs a uniform softening (eps)")
+        self.sysm, self.every, self.k = sysm, int(checkpoint_every), float(k)
+        self.dt = None if dt is None else float(dt)
+
+    def forward(self, steps: int) -> float:
+        s = self.sysm
+        self.steps = int(steps)
+        self.m0 = s.s["m"].clone()
+        self.istar0 = s.istar.clone()
+        self.init_state = _state(s)
+        self.init_rec = {}
+        s.initialize(record=self.init_rec)
+        self.cks = {}
+        self.dts = []
+        for n in range(self.steps):
+            if n % self.every == 0:
+                self.cks[n] = _save(s)
+            s.kdk(dt=self.dt)
+            self.dts.append(s.dt)
+        self.final = _state(s)
+        self.m_final = s.s["m"].clone()
+        return float(smoothed_unbound_t(self.final, self.m_final, self.k))
+
+    def backward(self) -> dict:
+        s = self.sysm
+        S = {k: v.clone().requires_grad_(True) for k, v in self.final.items()}
+        loss = smoothed_unbound_t(S, self.m_final, self.k)
+        cot = dict(zip(S, torch.autograd.grad(loss, list(S.values()), allow_unused=True)))
+        cot = {k: torch.zeros_like(S[k]) if v is None else v for k, v in cot.items()}
+        for n0 in sorted(self.cks, reverse=True):
+            n1 = min(n0 + self.every, self.steps)
+            _restore(s, self.cks[n0])
+            seg = []
+            for n in range(n0, n1):
+                st, m, rec = _state(s), s.s["m"].clone(), {}
+                s.kdk(dt=self.dts[n], record=rec)
+                seg.append((st, m, rec, self.dts[n]))
+            for st, m, rec, dt in reversed(seg):
+                inp = {k: v.clone().requires_grad_(True) for k, v in st.items()}
+                out = step_t(inp, m, rec, s, dt)
+                cot = _pull(out, inp, cot)
+                del out, inp
+        inp = {k: self.init_state[k].clone().requires_grad_(True)
+               for k in ("x", "y", "z", "vx", "vy", "vz", "uene", "alph", "alphu", "ksr")}
+        out = init_t(inp, self.m0, self.init_rec, s)
+        return _pull(out, inp, cot)
+
+
+def head_on_gradient(n_each: int = 100_000, steps: int = 200, dt: float | None = 1e-3, k: float = 20.0,
+                     eta: float = 1.18, eps: float = 0.01, theta: float = 0.5, checkpoint_every: int = 20,
+                     n_group: int = 32, separation: float = 3.0) -> tuple[float, np.ndarray, list]:
+    """Smoothed unbound mass after `steps` KDK steps of the head-on
+    collision of two n=1.5 stars (research_codes_amd.sensitivity.head_on_loss
+    at dx0 = dv0 = 0, FP64), its gradient with respect to the impactor's
+    offset dx0 and velocity change dv0 along the collision axis, and the
+    step lengths (dt, or the power-of-two rule's with dt=None)."""
+    p = ic.head_on(n_each, eta=eta, separation=separation)
+    with Context(0) as c:
+        s = SPHSystem(p, c, eta=eta, eps=eps, theta=theta, precision=FP64, n_group=n_group)
+        run = CollisionRun(s, dt, checkpoint_every, k)
+        loss = run.forward(steps)
+        g = run.backward()
+        imp = (run.istar0 == 1).double()
+        grad = np.array([float((g["x"] * imp).sum()), float((g["vx"] * imp).sum())])
+    return loss, grad, run.dts

@@ -88,6 +88,8 @@ class SPHSystem:
         self.dt = self.phys.dt_max
         if hmax is None:
             hmax = self.system_size() if len(counts) == 1 else 1e30
+        self.hmax = float(hmax)
+        self.last_perm = None
         # a constant softening makes every eps2 equal: one rsqrt per gravity pair
         ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax, uniform_softening=eps is not None)
         self.lists = {}
@@ -118,6 +120,7 @@ class SPHSystem:
         r2 = sum((self.s[c] - x0) ** 2 for c, x0 in zip(("x", "y", "z"), c0))
         h = float(torch.sqrt((d * r2).sum() / d.sum()))
         self.ctx.set_params(hmax=h)
+        self.hmax = h
         return h
 
     # ------------------------------------------------------------------ lists
@@ -129,7 +132,9 @@ class SPHSystem:
 
     def resort(self):
         """Re-establish Morton order after particles moved (permutes the state;
-        each segment of an ensemble on its own)."""
+        each segment of an ensemble on its own).  The permutation applied is
+        kept in `last_perm` (None when the order did not change)."""
+        self.last_perm = None
         if self.tree == "device":
             x, y, z = self.s["x"], self.s["y"], self.s["z"]
             pt = torch.cat([morton_order_dev(x[a:b], y[a:b], z[a:b]) + a for a, b in self._segments()])
@@ -141,6 +146,7 @@ class SPHSystem:
             if np.array_equal(perm, np.arange(self.n, dtype=perm.dtype)):
                 return
             pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
+        self.last_perm = pt.long()
         for k in L.STATE_FIELDS:
             self.s[k].copy_(self.s[k][pt])
         for t in (self.id, self.istar, self.rs, self.np_, self.itr, self.eps2):
@@ -449,23 +455,28 @@ class SPHSystem:
                 dt *= 2.0
         return dt
 
-    def initialize(self):
+    def initialize(self, record: dict | None = None):
         """startSimulation's first force evaluation (hdr_hgas.hpp:1335-1378)."""
+        hmax = self.hmax
         self.converge_density()
         self.eos_balsara()
         self.build_lists(("hydro", "grav"))
         self.gravity()
         self.hydro()
+        if record is not None:
+            record.update(perm=None, hmax=hmax, dens=self.lists["dens"], hydro=self.lists["hydro"])
         self.update_hmax()
 
-    def kdk(self, dt: float | None = None):
+    def kdk(self, dt: float | None = None, record: dict | None = None):
         """One loopSimulation step (hdr_hgas.hpp:1499-1539) with a fixed list
         rebuild per step.  dt overrides the power-of-two timestep rule (a
-        fixed schedule, e.g. for finite-difference sensitivities).  Returns
-        the diagnostics before the step."""
+        fixed schedule, e.g. for finite-difference sensitivities).  `record`
+        (a dict) receives the step's reorder, H_max and lists.  Returns the
+        diagnostics before the step."""
         d = self.diagnostics()
         self.dt = self.next_dt(d["dt_min"]) if dt is None else float(dt)
         dt = self.dt
+        hmax = self.hmax
         check(lib().sphk_predict_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_predict_dev")
         self.resort()
@@ -474,6 +485,9 @@ class SPHSystem:
         self.build_lists(("hydro", "grav"))
         self.gravity()
         self.hydro()
+        if record is not None:
+            record.update(perm=self.last_perm, hmax=hmax, dens=self.lists["dens"],
+                          hydro=self.lists["hydro"])
         self.update_hmax()
         check(lib().sphk_correct_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_correct_dev")
