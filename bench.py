@@ -414,7 +414,8 @@ def adjoint_main(args) -> int:
         loss = run.forward(args.steps)
         torch.cuda.synchronize()
         fwd = time.perf_counter() - t
-        log(f"forward {args.steps} steps in {fwd:.2f}s, t_end={s.time:.4g}, loss={loss:.6e}")
+        t_end = s.time
+        log(f"forward {args.steps} steps in {fwd:.2f}s, t_end={t_end:.4g}, loss={loss:.6e}")
         t = time.perf_counter()
         g = run.backward()
         torch.cuda.synchronize()
@@ -423,7 +424,7 @@ def adjoint_main(args) -> int:
         imp = (run.istar0 == 1).double()
         grad = np.array([float((g["x"] * imp).sum()), float((g["vx"] * imp).sum())])
         log(f"backward in {bwd:.2f}s, gradient {grad}, peak {peak:.1f} GiB")
-        dts, t_end = list(run.dts), s.time
+        dts = list(run.dts)
         del run, g, s
     t = time.perf_counter()
     fd = fd_gradient(lambda a, b: head_on_loss(a, b, n_each=n_each, eta=args.eta, precision=FP64,

@@ -71,8 +71,69 @@ def _sqrt0(s):
     return torch.where(pos, torch.sqrt(torch.where(pos, s, torch.ones_like(s))), torch.zeros_like(s))
 
 
-def _sum(n, idx, v):
-    return torch.zeros(n, dtype=v.dtype, device=v.device).index_add(0, idx, v)
+class _SegSum(torch.autograd.Function):
+    """Per-particle sums of pair values whose particle indices are sorted
+    (a segmented reduction: no atomics, unlike index_add over pairs that
+    share their i); the gradient is the gather of the sums' cotangent."""
+
+    @staticmethod
+    def forward(ctx, v, idx, lengths):
+        ctx.save_for_backward(idx)
+        return torch.segment_reduce(v, "sum", lengths=lengths, unsafe=True, initial=0.0)
+
+    @staticmethod
+    def backward(ctx, g):
+        (idx,) = ctx.saved_tensors
+        return g[idx], None, None
+
+
+class _GatherSorted(torch.autograd.Function):
+    """x[idx] for sorted idx; the gradient is a segmented sum."""
+
+    @staticmethod
+    def forward(ctx, x, idx, lengths):
+        ctx.save_for_backward(lengths)
+        return x[idx]
+
+    @staticmethod
+    def backward(ctx, g):
+        (lengths,) = ctx.saved_tensors
+        return torch.segment_reduce(g, "sum", lengths=lengths, unsafe=True, initial=0.0), None, None
+
+
+class _Gather(torch.autograd.Function):
+    """x[idx]; the gradient is scattered with atomic adds (no sort)."""
+
+    @staticmethod
+    def forward(ctx, x, idx):
+        ctx.save_for_backward(idx)
+        ctx.n = x.numel()
+        return x[idx]
+
+    @staticmethod
+    def backward(ctx, g):
+        (idx,) = ctx.saved_tensors
+        return torch.zeros(ctx.n, dtype=g.dtype, device=g.device).index_add_(0, idx, g), None
+
+
+class _Seg:
+    """A pair set (i, j) sorted by i: per-i sums and the gathers of the two
+    sides, with gradients that avoid sorting the indices."""
+
+    def __init__(self, i, j, n: int):
+        if i.numel() > 1 and not bool((i[1:] >= i[:-1]).all()):
+            raise ValueError("pairs must be sorted by their i index")
+        self.i, self.j = i, j
+        self.lengths = torch.bincount(i, minlength=n)
+
+    def sum(self, v):
+        return _SegSum.apply(v, self.i, self.lengths)
+
+    def gi(self, x):
+        return _GatherSorted.apply(x, self.i, self.lengths) if x.requires_grad else x[self.i]
+
+    def gj(self, x):
+        return _Gather.apply(x, self.j) if x.requires_grad else x[self.j]
 
 
 def expand_pairs(w: Walks) -> tuple[torch.Tensor, torch.Tensor]:
@@ -104,8 +165,8 @@ def _density_t(P, m, pairs, hmax, eta, ksrh):
     rotv and grdh (= Omega^-1), differentiable in the positions, velocities
     and the starting H (P["ksr"])."""
     i, j = pairs
-    n = m.numel()
-    dx, dy, dz = (P[c][i] - P[c][j] for c in ("x", "y", "z"))
+    seg = _Seg(i, j, m.numel())
+    dx, dy, dz = (seg.gi(P[c]) - seg.gj(P[c]) for c in ("x", "y", "z"))
     r2 = dx * dx + dy * dy + dz * dz
     r = _sqrt0(r2)
     mj = m[j]
@@ -113,21 +174,21 @@ def _density_t(P, m, pairs, hmax, eta, ksrh):
     dens = None
     for _ in range(3):                                  # hdr_density.hpp:264-302
         hi = 1.0 / H
-        dens = _sum(n, i, mj * _w0(r * hi[i])) * (hi * hi * hi)
+        dens = seg.sum(mj * _w0(r * seg.gi(hi))) * (hi * hi * hi)
         H = torch.clamp(eta * ksrh * (m / dens) ** (1.0 / 3.0), max=hmax)
     hi = 1.0 / H
     hi4 = hi * hi * hi * hi
     self_ = i == j                                      # ids are unique: self = same index
     ri = torch.where(self_, torch.zeros_like(r), 1.0 / torch.where(self_, torch.ones_like(r), r))
-    q = r2 * ri * hi[i]
+    q = r2 * ri * seg.gi(hi)
     kw0, kw1 = _w0(q), _w1(q)
-    grdh = -_sum(n, i, (3.0 * kw0 + q * kw1) * mj) * hi4
-    dw = mj * hi4[i] * kw1 * ri
-    dvx, dvy, dvz = (P[c][i] - P[c][j] for c in ("vx", "vy", "vz"))
-    divv = -_sum(n, i, (dvx * dx + dvy * dy + dvz * dz) * dw)
-    rx = _sum(n, i, (dvy * dz - dvz * dy) * dw)
-    ry = _sum(n, i, (dvz * dx - dvx * dz) * dw)
-    rz = _sum(n, i, (dvx * dy - dvy * dx) * dw)
+    grdh = -seg.sum((3.0 * kw0 + q * kw1) * mj) * hi4
+    dw = mj * seg.gi(hi4) * kw1 * ri
+    dvx, dvy, dvz = (seg.gi(P[c]) - seg.gj(P[c]) for c in ("vx", "vy", "vz"))
+    divv = -seg.sum((dvx * dx + dvy * dy + dvz * dz) * dw)
+    rx = seg.sum((dvy * dz - dvz * dy) * dw)
+    ry = seg.sum((dvz * dx - dvx * dz) * dw)
+    rz = seg.sum((dvx * dy - dvy * dx) * dw)
     omg = 1.0 / (1.0 + H / dens * grdh / 3.0)           # hdr_density.hpp:351-362
     rot = _sqrt0(rx * rx + ry * ry + rz * rz)
     return dens, H, divv / dens * omg, rot / dens * omg, omg
@@ -138,13 +199,13 @@ def _density_radius(P, m, pairs, hmax, eta, ksrh):
     entries beyond it contribute nothing to any pass."""
     with torch.no_grad():
         i, j = pairs
-        n = m.numel()
+        seg = _Seg(i, j, m.numel())
         r = torch.sqrt(sum((P[c][i] - P[c][j]) ** 2 for c in ("x", "y", "z")))
         H = P["ksr"].detach()
         R = H.clone()
         for _ in range(3):
             hi = 1.0 / H
-            dens = _sum(n, i, m[j] * _w0(r * hi[i])) * (hi * hi * hi)
+            dens = seg.sum(m[j] * _w0(r * hi[i])) * (hi * hi * hi)
             H = torch.clamp(eta * ksrh * (m / dens) ** (1.0 / 3.0), max=hmax)
             R = torch.maximum(R, H)
         return r, R
@@ -153,33 +214,34 @@ def _density_radius(P, m, pairs, hmax, eta, ksrh):
 def _hydro_t(P, m, pairs, d, G):
     """The hydro functor on non-self pairs (i, j) with r < max(H_i, H_j)."""
     i, j = pairs
-    n = m.numel()
-    dx, dy, dz = (P[c][i] - P[c][j] for c in ("x", "y", "z"))
-    dvx, dvy, dvz = (P[c][i] - P[c][j] for c in ("vx", "vy", "vz"))
+    seg = _Seg(i, j, m.numel())
+    gi, gj = seg.gi, seg.gj
+    dx, dy, dz = (gi(P[c]) - gj(P[c]) for c in ("x", "y", "z"))
+    dvx, dvy, dvz = (gi(P[c]) - gj(P[c]) for c in ("vx", "vy", "vz"))
     r = torch.sqrt(dx * dx + dy * dy + dz * dz)
     ri = 1.0 / r
     hi = d["hinv"]
     hi4 = hi * hi * hi * hi
-    dwi = hi4[i] * _w1(r * hi[i])
-    dwj = hi4[j] * _w1(r * hi[j])
-    ka = (dwi + dwj) * m[j]
+    dwi = gi(hi4) * _w1(r * gi(hi))
+    dwj = gj(hi4) * _w1(r * gj(hi))
+    mj = m[j]
+    ka = (dwi + dwj) * mj
     wij = (dx * dvx + dy * dvy + dz * dvz) * ri
     w0 = torch.clamp(wij, max=0.0)
-    vs = d["vsnd"][i] + d["vsnd"][j] - 3.0 * w0
-    rhi = 1.0 / (d["dens"][i] + d["dens"][j])
-    av0 = (d["bswt_half"][i] + d["bswt_half"][j]) * (P["alph"][i] + P["alph"][j]) * vs * w0
-    pc = d["presc"]
-    ta = (pc[i] + pc[j] - 0.5 * av0 * rhi) * ka * ri
-    vsu = _sqrt0(torch.abs(d["pres"][i] - d["pres"][j]) * rhi * 2.0)
-    du = d["thrm"][i] - d["thrm"][j]
-    udot = _sum(n, i, ka * (pc[i] * wij - rhi * (0.25 * av0 * w0 - (P["alphu"][i] + P["alphu"][j]) * vsu * du)))
-    et = d["eta"]
-    dg = m[j] * ri * (et[i] * dwi + et[j] * dwj)
-    out = {"achx": -0.5 * _sum(n, i, ta * dx), "achy": -0.5 * _sum(n, i, ta * dy),
-           "achz": -0.5 * _sum(n, i, ta * dz), "udot": 0.5 * udot,
-           "diffu": 2.0 * _sum(n, i, rhi * du * ka * ri)}
+    vs = gi(d["vsnd"]) + gj(d["vsnd"]) - 3.0 * w0
+    rhi = 1.0 / (gi(d["dens"]) + gj(d["dens"]))
+    av0 = (gi(d["bswt_half"]) + gj(d["bswt_half"])) * (gi(P["alph"]) + gj(P["alph"])) * vs * w0
+    pci = gi(d["presc"])
+    ta = (pci + gj(d["presc"]) - 0.5 * av0 * rhi) * ka * ri
+    vsu = _sqrt0(torch.abs(gi(d["pres"]) - gj(d["pres"])) * rhi * 2.0)
+    du = gi(d["thrm"]) - gj(d["thrm"])
+    udot = seg.sum(ka * (pci * wij - rhi * (0.25 * av0 * w0 - (gi(P["alphu"]) + gj(P["alphu"])) * vsu * du)))
+    dg = mj * ri * (gi(d["eta"]) * dwi + gj(d["eta"]) * dwj)
+    out = {"achx": -0.5 * seg.sum(ta * dx), "achy": -0.5 * seg.sum(ta * dy),
+           "achz": -0.5 * seg.sum(ta * dz), "udot": 0.5 * udot,
+           "diffu": 2.0 * seg.sum(rhi * du * ka * ri)}
     for c, dc in (("x", dx), ("y", dy), ("z", dz)):
-        out["g1" + c] = G * 0.5 * _sum(n, i, dg * dc)
+        out["g1" + c] = G * 0.5 * seg.sum(dg * dc)
     return out
 
 
