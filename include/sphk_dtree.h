@@ -42,6 +42,13 @@ int sphk_morton_order_dev(int n, const double *x, const double *y, const double 
  * with SPHK_E_ARG if the particles are not in Morton order. */
 int sphk_dtree_build(int n, const double *x, const double *y, const double *z, const double *m,
                      const double *eps2, int n_leaf, int n_group, void *stream, sphk_dtree **out);
+/* The same build into an existing handle, reusing its device buffers (they
+ * only grow), so that a tree rebuilt every step allocates nothing once its
+ * buffers have reached the run's sizes.  Arrays returned by the handle
+ * before are invalid afterwards.  On failure the handle holds no tree
+ * (n_walk 0) and may only be rebuilt or freed. */
+int sphk_dtree_rebuild(sphk_dtree *t, int n, const double *x, const double *y, const double *z,
+                       const double *m, const double *eps2, int n_leaf, int n_group, void *stream);
 void sphk_dtree_free(sphk_dtree *t);
 
 int sphk_dtree_n_walk(const sphk_dtree *t);

@@ -119,6 +119,7 @@ SIGNATURES.update({
     # sphk_dtree.h (all array arguments are device pointers)
     "sphk_morton_order_dev": (C.c_int, [C.c_int, _P, _P, _P, _P, _P]),
     "sphk_dtree_build": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)]),
+    "sphk_dtree_rebuild": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P]),
     "sphk_dtree_free": (None, [_P]),
     "sphk_dtree_n_walk": (C.c_int, [_P]),
     "sphk_dtree_n_node": (C.c_int, [_P]),

@@ -980,22 +980,28 @@ int sphk_morton_order_dev(int n, const double *x, const double *y, const double 
     return rc;
 }
 
-int sphk_dtree_build(int n, const double *x, const double *y, const double *z, const double *m,
-                     const double *eps2, int n_leaf, int n_group, void *stream, sphk_dtree **out)
+namespace {
+
+// The build proper, into t's buffers (grown as needed).  On failure t holds
+// no tree; its buffers stay allocated for a later build or sphk_dtree_free.
+int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const double *z, const double *m,
+               const double *eps2, int n_leaf, int n_group, hipStream_t stream)
 {
-    if (!out) return set_error(SPHK_E_ARG, "dtree: null output");
-    *out = nullptr;
-    if (n <= 0 || !x || !y || !z || !m) return set_error(SPHK_E_ARG, "dtree: bad particle arrays");
-    if (n_leaf < 1 || n_group < 1) return set_error(SPHK_E_ARG, "dtree: n_leaf, n_group must be >= 1");
-    sphk_dtree *t = new (std::nothrow) sphk_dtree();
-    if (!t) return set_error(SPHK_E_CAPACITY, "dtree: out of memory");
+    // a previous user of the buffers may still be queued on the old stream
+    if (t->st != stream && hipStreamSynchronize(t->st) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "dtree: stream synchronisation failed");
     t->n = n;
     t->n_leaf = n_leaf;
     t->n_group = n_group;
-    t->st = (hipStream_t)stream;
+    t->st = stream;
+    t->n_walk = 0;
+    t->n_node = 0;
+    t->n_front = 0;
+    for (auto &L : t->lists) L.built = false;
     auto fail = [&](int rc) {
-        t->release();
-        delete t;
+        t->n_walk = 0;
+        t->n_node = 0;
+        t->n_front = 0;
         return rc;
     };
     hipStream_t st = t->st;
@@ -1107,8 +1113,43 @@ int sphk_dtree_build(int n, const double *x, const double *y, const double *z, c
     hipLaunchKernelGGL(front_write_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64c.p, t->front.p);
     if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
         return fail(set_error(SPHK_E_DEVICE, "dtree: frontier failed"));
+    return SPHK_OK;
+}
+
+int check_build_args(int n, const double *x, const double *y, const double *z, const double *m, int n_leaf,
+                     int n_group)
+{
+    if (n <= 0 || !x || !y || !z || !m) return set_error(SPHK_E_ARG, "dtree: bad particle arrays");
+    if (n_leaf < 1 || n_group < 1) return set_error(SPHK_E_ARG, "dtree: n_leaf, n_group must be >= 1");
+    return SPHK_OK;
+}
+
+}  // namespace
+
+int sphk_dtree_build(int n, const double *x, const double *y, const double *z, const double *m,
+                     const double *eps2, int n_leaf, int n_group, void *stream, sphk_dtree **out)
+{
+    if (!out) return set_error(SPHK_E_ARG, "dtree: null output");
+    *out = nullptr;
+    if (int rc = check_build_args(n, x, y, z, m, n_leaf, n_group)) return rc;
+    sphk_dtree *t = new (std::nothrow) sphk_dtree();
+    if (!t) return set_error(SPHK_E_CAPACITY, "dtree: out of memory");
+    t->st = (hipStream_t)stream;
+    if (int rc = build_tree(t, n, x, y, z, m, eps2, n_leaf, n_group, (hipStream_t)stream)) {
+        t->release();
+        delete t;
+        return rc;
+    }
     *out = t;
     return SPHK_OK;
+}
+
+int sphk_dtree_rebuild(sphk_dtree *t, int n, const double *x, const double *y, const double *z,
+                       const double *m, const double *eps2, int n_leaf, int n_group, void *stream)
+{
+    if (!t) return set_error(SPHK_E_ARG, "dtree: null handle");
+    if (int rc = check_build_args(n, x, y, z, m, n_leaf, n_group)) return rc;
+    return build_tree(t, n, x, y, z, m, eps2, n_leaf, n_group, (hipStream_t)stream);
 }
 
 void sphk_dtree_free(sphk_dtree *t)
@@ -1126,12 +1167,14 @@ const int32_t *sphk_dtree_i_off(const sphk_dtree *t) { return t ? t->i_off.p : n
 int sphk_dtree_list_gather(sphk_dtree *t, const double *rs)
 {
     if (!t || !rs) return set_error(SPHK_E_ARG, "dtree gather: null argument");
+    if (t->n_walk <= 0) return set_error(SPHK_E_STATE, "dtree gather: no tree built");
     return build_list<0>(t, rs, 0.);
 }
 
 int sphk_dtree_list_symmetry(sphk_dtree *t, const double *h)
 {
     if (!t || !h) return set_error(SPHK_E_ARG, "dtree symmetry: null argument");
+    if (t->n_walk <= 0) return set_error(SPHK_E_STATE, "dtree symmetry: no tree built");
     const size_t nn = (size_t)t->n_node;
     if (t->olo.reserve(3 * nn, false, t->st) != hipSuccess || t->ohi.reserve(3 * nn, false, t->st) != hipSuccess)
         return set_error(SPHK_E_DEVICE, "dtree symmetry: allocation failed");
@@ -1155,6 +1198,7 @@ int sphk_dtree_list_symmetry(sphk_dtree *t, const double *h)
 int sphk_dtree_list_long(sphk_dtree *t, double theta)
 {
     if (!t || !(theta >= 0.)) return set_error(SPHK_E_ARG, "dtree long: bad argument");
+    if (t->n_walk <= 0) return set_error(SPHK_E_STATE, "dtree long: no tree built");
     const double inv_t2 = theta > 0. ? 1. / (theta * theta) : std::numeric_limits<double>::infinity();
     return build_list<2>(t, nullptr, inv_t2);
 }

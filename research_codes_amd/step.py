@@ -94,6 +94,7 @@ class SPHSystem:
         ctx.set_params(eta=eta, ksrh=KSRH, hmax=hmax, uniform_softening=eps is not None)
         self.lists = {}
         self.list_build_s = 0.0
+        self._dtrees = {}     # device trees per (segment, group size), rebuilt in place
         self._ss = L.State(**{k: self.s[k].data_ptr() for k in L.STATE_FIELDS})
 
     # ---------------------------------------------------------- H_max (glue)
@@ -164,8 +165,13 @@ class SPHSystem:
         """Tree over particles [a, b) with walk group size g."""
         s = self.s
         if self.tree == "device":
-            return DeviceTree(s["x"][a:b], s["y"][a:b], s["z"][a:b], s["m"][a:b], self.eps2[a:b],
-                              self.n_leaf, g)
+            args = (s["x"][a:b], s["y"][a:b], s["z"][a:b], s["m"][a:b], self.eps2[a:b], self.n_leaf, g)
+            t = self._dtrees.get((a, b, g))
+            if t is None:
+                t = self._dtrees[(a, b, g)] = DeviceTree(*args)
+            else:
+                t.rebuild(*args)
+            return t
         x, y, z, m = (s[k][a:b].cpu().numpy() for k in ("x", "y", "z", "m"))
         return HostTree(x, y, z, m, self.eps2[a:b].cpu().numpy(), self.n_leaf, g)
 
@@ -181,6 +187,7 @@ class SPHSystem:
 
         parts = {k: [] for k in kinds}
         moms = []
+        dev_tree = self.tree == "device"
         for a, b in self._segments():
             trees = {}
             try:
@@ -189,7 +196,6 @@ class SPHSystem:
                     if g not in trees:
                         trees[g] = self._tree(a, b, g)
                     return trees[g]
-                dev_tree = self.tree == "device"
                 if "dens" in kinds:
                     t = tree_for("dens")
                     rs = self.rs[a:b] if dev_tree else self.rs[a:b].cpu().numpy()
@@ -203,8 +209,9 @@ class SPHSystem:
                     parts["grav"].append((a, t.i_off, *t.long(self.theta)))
                     moms.append({k: t32(v) for k, v in t.moments().items()})
             finally:
-                for t in trees.values():
-                    t.close()
+                if not dev_tree:
+                    for t in trees.values():
+                        t.close()
         for kind, pl in parts.items():
             if len(pl) == 1:
                 a, i_off, *rest = pl[0]

@@ -163,17 +163,32 @@ class DeviceTree:
     the same device (copies owned by the caller)."""
 
     def __init__(self, x, y, z, m, eps2=None, n_leaf: int = 8, n_group: int = 64):
-        import torch
         h = _P()
         n = x.numel()
+        self.n = n
         check(lib().sphk_dtree_build(n, _dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(eps2),
                                      int(n_leaf), int(n_group), _stream(), C.byref(h)),
               "sphk_dtree_build")
         self._h = h
-        self.n = n
+        self._read_walks()
+
+    def _read_walks(self):
+        import torch
+        h = self._h
         self.n_walk = lib().sphk_dtree_n_walk(h)
         self.n_node = lib().sphk_dtree_n_node(h)
         self.i_off = _copy_out(lib().sphk_dtree_i_off(h), self.n_walk + 1, torch.int32)
+
+    def rebuild(self, x, y, z, m, eps2=None, n_leaf: int = 8, n_group: int = 64):
+        """Build again for new particles into this handle's device buffers
+        (no allocation once they have reached the run's sizes)."""
+        if not self._h:
+            raise RuntimeError("tree is closed")
+        self.n = x.numel()
+        check(lib().sphk_dtree_rebuild(self._h, self.n, _dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(eps2),
+                                       int(n_leaf), int(n_group), _stream()), "sphk_dtree_rebuild")
+        self._read_walks()
+        return self
 
     def close(self):
         if self._h:

@@ -76,6 +76,38 @@ def test_tree_and_lists_match_host(case, n_leaf, n_group):
         dt.close()
 
 
+def test_rebuild_in_place_matches_fresh_build():
+    """One handle rebuilt for a sequence of particle sets (larger, smaller,
+    another leaf and group size) gives exactly what a new handle gives."""
+    cases = _cases()
+    seq = [(sort_particles(cases["star"])[0], 8, 32), (sort_particles(cases["pair"])[0], 8, 16),
+           (sort_particles(cases["star"])[0], 4, 64)]
+    seq.append((sort_particles({k: v[:1200] for k, v in cases["pair"].items()})[0], 8, 32))
+    keep = None
+    for p, n_leaf, n_group in seq:
+        args = [_d(p[k]) for k in ("x", "y", "z", "m")] + [_d(p["ksr"] ** 2), n_leaf, n_group]
+        print(f"[dtree] rebuild n={len(p['x'])} n_leaf={n_leaf} n_group={n_group}", flush=True)
+        if keep is None:
+            keep = DeviceTree(*args)
+        else:
+            keep.rebuild(*args)
+        fresh = DeviceTree(*args)
+        try:
+            assert keep.n_walk == fresh.n_walk and keep.n_node == fresh.n_node
+            assert torch.equal(keep.i_off, fresh.i_off)
+            for k in ("m", "x", "y", "z", "eps2"):
+                assert torch.equal(keep.moments()[k], fresh.moments()[k]), k
+            rs = _d(1.1 * p["ksr"])
+            for name, a, b in (("gather", keep.gather(rs), fresh.gather(rs)),
+                               ("symmetry", keep.symmetry(_d(p["ksr"])), fresh.symmetry(_d(p["ksr"]))),
+                               ("long", keep.long(0.5), fresh.long(0.5))):
+                for u, v in zip(a, b):
+                    assert torch.equal(u, v), name
+        finally:
+            fresh.close()
+    keep.close()
+
+
 def test_step_with_device_lists_equals_host_lists():
     """The same lists give bitwise the same force step."""
     p = ic.head_on(3000, eta=1.18)
