@@ -392,6 +392,12 @@ class CollisionRun:
     def __init__(self, sysm: SPHSystem, dt: float | None, checkpoint_every: int = 20, k: float = 20.0):
         if sysm.eps is None:
             raise ValueError("the gravity adjoint needs a uniform softening (eps)")
+        if sysm.precision != FP64:
+            raise ValueError("the restated step is FP64: run the system in FP64 precision")
+        if len(sysm.seg) != 2:
+            raise ValueError("one run per system: differentiate the runs of an ensemble one by one")
+        if sysm.tree != "device":
+            raise ValueError("the gravity adjoint walks the device tree")
         self.sysm, self.every, self.k = sysm, int(checkpoint_every), float(k)
         self.dt = None if dt is None else float(dt)
 
