@@ -88,3 +88,16 @@ def test_long_rollout_gradient_matches_finite_differences():
     rel = np.abs(g - fd) / np.abs(fd).max()
     print(f"[sph-adjoint] 200 steps to t={sum(dts):.4g}: loss {loss:.6e}, adjoint {g} vs FD {fd}, rel {rel}")
     assert loss > 1e-3 and np.all(rel <= 1e-6)
+
+
+def test_collision_run_rejects_unsupported_systems(ctx):
+    from research_codes_amd.device import MIXED
+    from research_codes_amd.sph_adjoint import CollisionRun
+    p = ic.head_on(300, eta=1.18)
+    with pytest.raises(ValueError, match="FP64"):
+        CollisionRun(SPHSystem(p, ctx, eta=1.18, eps=0.01, precision=MIXED), DT)
+    with pytest.raises(ValueError, match="softening"):
+        CollisionRun(SPHSystem(p, ctx, eta=1.18, precision=FP64), DT)
+    n = len(p["x"])
+    with pytest.raises(ValueError, match="ensemble"):
+        CollisionRun(SPHSystem(p, ctx, eta=1.18, eps=0.01, precision=FP64, segments=[n // 2, n - n // 2]), DT)
