@@ -77,6 +77,8 @@ __device__ __forceinline__ double cbrt_f64(double x)
 __device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_r(double x) { return rsqrt_f64(x); }
 __device__ __forceinline__ float rcp_r(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float abs_r(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double abs_r(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ double rcp_r(double x) { return rcp_f64(x); }
 
 // W(q) = C0 (1-q)_+^4 (1+4q), hdr_kernel.hpp:246-250.

@@ -149,11 +149,14 @@ __device__ __forceinline__ void dens_pass2_pair(const double4 &j, const R4<R> &v
     const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
     const R ri = (id_i != unpack_id(v.d)) ? rsqrt_r(r2) : R(0);
     const R q = r2 * ri * hi;
-    const R kw0 = kernel0th(q);
-    const R kw1 = kernel1st(q);
+    // with t = (1-q)_+: W' = -5 C1 q t^3, 3W + qW' = C0 t^3 (3 + 9q - 32q^2),
+    // and W'(q) / r = -5 C1 h t^3 (the self entry has d = 0: no div/curl term)
+    R t = R(1) - q;
+    t = t > R(0) ? t : R(0);
+    const R t3 = t * t * t;
     const R mh = (R)j.w * hi4;
-    a.g -= (R(3) * kw0 + q * kw1) * mh;
-    const R dw = mh * kw1 * ri;
+    a.g -= R(kCeff0) * t3 * (R(3) + q * (R(9) - R(32) * q)) * mh;
+    const R dw = R(-5. * kCeff1) * mh * hi * t3;
     const R dwx = dw * dpx, dwy = dw * dpy, dwz = dw * dpz;
     a.dv -= dvx * dwx + dvy * dwy + dvz * dwz;
     a.rx += dvy * dwz - dvz * dwy;
@@ -655,7 +658,7 @@ struct HPos<float> {
 template <typename R>
 struct HydroI {
     double px, py, pz;   // FP64 path: absolute; mixed path: offsets (in the FP32 range)
-    R vx, vy, vz, hi, hi4, pp, pc, bs, dn, cs, al, au, eg, et;
+    R vx, vy, vz, hi, h5, pp, pc, bs, dn, cs, al, au, eg, et;
     int32_t id;
 };
 
@@ -678,16 +681,26 @@ __device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const
     const R dvy = I.vy - A.b;
     const R dvz = I.vz - A.c;
     const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-    const R ri = (I.id != jid) ? rsqrt_r(r2) : R(0);
+    const bool other = I.id != jid;
+    const R ri = other ? rsqrt_r(r2) : R(0);
     const R r1 = r2 * ri;
     const R hj = A.d;
     const R q_i = r1 * I.hi;
     const R q_j = r1 * hj;
-    const R hi4_j = hj * (hj * hj * hj);
-    const R dw_i = I.hi4 * kernel1st(q_i);
-    const R dw_j = hi4_j * kernel1st(q_j);
+    // W'(q) = -5 C1 q (1-q)_+^3, so dW_k / r = h_k^4 W'(r h_k) / r = -5 C1 F_k
+    // with F_k = h_k^5 (1 - q_k)_+^3: ka/r and the eta term need no kernel
+    // evaluation beyond F (the self entry contributes nothing, as ri = 0
+    // makes it in the reference's form)
+    R ti = R(1) - q_i, tj = R(1) - q_j;
+    ti = ti > R(0) ? ti : R(0);
+    tj = tj > R(0) ? tj : R(0);
+    const R hj2 = hj * hj;
+    const R Fi = I.h5 * (ti * ti * ti);
+    const R Fj = (hj2 * hj2 * hj) * (tj * tj * tj);
     const R m_j = (R)p.w;
-    const R ka = (dw_i + dw_j) * m_j;
+    const R mk = other ? R(-5. * kCeff1) * m_j : R(0);
+    const R kri = mk * (Fi + Fj);   // ka * ri, ka = (dW_i + dW_j) m_j
+    const R ka = kri * r1;
 
     const R rv = dpx * dvx + dpy * dvy + dpz * dvz;
     const R w_ij = rv * ri;
@@ -697,19 +710,18 @@ __device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const
     const R av0 = (I.bs + B.c) * (I.al + Cc.b) * vs * w0;
     a.vs = a.vs > vs ? a.vs : vs;
 
-    const R kri = ka * ri;
     const R ta = (I.pc + B.b - R(0.5) * av0 * rhi) * kri;
     a.ax -= ta * dpx;
     a.ay -= ta * dpy;
     a.az -= ta * dpz;
 
-    const R vsu2 = __builtin_fabs(I.pp - B.a) * rhi * R(2);
+    const R vsu2 = abs_r(I.pp - B.a) * rhi * R(2);
     const R vsu = vsu2 != R(0) ? vsu2 * rsqrt_r(vsu2) : R(0);
     const R du = I.eg - Cc.d;
     a.ud += ka * (I.pc * w_ij - rhi * (R(0.25) * av0 * w0 - (I.au + Cc.c) * vsu * du));
     a.df += rhi * du * kri;
 
-    const R dg = m_j * ri * (I.et * dw_i + eta_j * dw_j);
+    const R dg = mk * (I.et * Fi + eta_j * Fj);   // m_j ri (eta_i dW_i + eta_j dW_j)
     a.gx += dg * dpx;
     a.gy += dg * dpy;
     a.gz += dg * dpz;
@@ -755,7 +767,7 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
         I.vz = (R)pi.vz[i];
         const double hid = pi.hinv[i];
         I.hi = (R)hid;
-        I.hi4 = (R)(hid * (hid * hid * hid));
+        I.h5 = (R)(hid * (hid * hid * hid) * hid);
         I.pp = (R)pi.pres[i];
         I.pc = (R)pi.presc[i];
         I.bs = (R)pi.bswt[i];
