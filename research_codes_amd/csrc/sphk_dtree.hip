@@ -415,19 +415,70 @@ __global__ __launch_bounds__(kBlock) void group_write_kernel(int n, const int32_
 }
 
 // ---- lists -------------------------------------------------------------------
+// Node records for the list walks: the fields one visit reads, packed in
+// one 64-byte record per node (one memory segment per visit instead of one
+// per field array).  Values are copies of the node arrays, so the walks'
+// comparisons and their results are unchanged.
+struct alignas(64) BoxRec {     // gather and symmetry walks
+    double lo[3], hi[3];        // tight box
+    int32_t child, nchild, begin, end;
+};
+struct alignas(64) MacRec {     // long walk
+    double mx, my, mz, mm, len;
+    int32_t child, nchild, begin, end;
+};
+struct alignas(64) OboxRec {    // symmetry walk: box of the nodes' particle spheres
+    double lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(kBlock) void pack_nodes_kernel(int nn, NodeArrays nd, Moments mo,
+                                                            BoxRec *__restrict__ br, MacRec *__restrict__ mr)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nn) return;
+    const size_t C = mo.cap;
+    BoxRec b;
+    for (int d = 0; d < 3; d++) {
+        b.lo[d] = mo.lo[d * C + a];
+        b.hi[d] = mo.hi[d * C + a];
+    }
+    b.child = nd.child[a];
+    b.nchild = nd.nchild[a];
+    b.begin = nd.begin[a];
+    b.end = nd.end[a];
+    br[a] = b;
+    mr[a] = MacRec{mo.mx[a], mo.my[a], mo.mz[a], mo.mm[a], nd.len[a], b.child, b.nchild, b.begin, b.end};
+}
+
+__global__ __launch_bounds__(kBlock) void pack_obox_kernel(int nn, const double *__restrict__ olo,
+                                                           const double *__restrict__ ohi, size_t C,
+                                                           OboxRec *__restrict__ orr)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nn) return;
+    OboxRec o;
+    for (int d = 0; d < 3; d++) {
+        o.lo[d] = olo[d * C + a];
+        o.hi[d] = ohi[d * C + a];
+    }
+    orr[a] = o;
+}
+
 struct TreeView {
     NodeArrays nd;
     Moments mo;
-    const double *olo, *ohi;
+    const BoxRec *br;
+    const MacRec *mr;
+    const OboxRec *orr;
     const double *x, *y, *z;
     const int32_t *group, *i_off;
 };
 
 __device__ __forceinline__ bool box_overlaps(const double (&alo)[3], const double (&ahi)[3],
-                                             const double *blo, const double *bhi, size_t C, int b)
+                                             const double (&blo)[3], const double (&bhi)[3])
 {
     for (int d = 0; d < 3; d++)
-        if (bhi[d * C + b] < alo[d] || blo[d * C + b] > ahi[d]) return false;
+        if (bhi[d] < alo[d] || blo[d] > ahi[d]) return false;
     return true;
 }
 
@@ -459,24 +510,35 @@ __global__ __launch_bounds__(kBlock) void walk_box_kernel(int ng, const int32_t 
 // the long list), take it as one SPJ, emit its particles (leaf), or open it.
 enum Visit { kSkip, kSpj, kLeaf, kOpen };
 
+// Child range and particle range of the visited node, from its record.
+struct NodeLinks {
+    int32_t child, nchild, begin, end;
+};
+
 template <int MODE>
-__device__ __forceinline__ Visit visit(const TreeView &t, size_t C, int a, const double (&olo)[3],
+__device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&olo)[3],
                                        const double (&ohi)[3], const double (&glo)[3], const double (&ghi)[3],
-                                       double inv_t2)
+                                       double inv_t2, NodeLinks &ln)
 {
 #pragma clang fp contract(off)
-    const bool leaf = t.nd.child[a] < 0;
-    if (MODE == 0) {
-        if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a)) return kSkip;
+    if (MODE != 2) {
+        const BoxRec b = t.br[a];
+        ln = NodeLinks{b.child, b.nchild, b.begin, b.end};
+        const bool leaf = b.child < 0;
+        if (MODE == 0) {
+            if (!box_overlaps(olo, ohi, b.lo, b.hi)) return kSkip;
+            return leaf ? kLeaf : kOpen;
+        }
+        if (!box_overlaps(olo, ohi, b.lo, b.hi)) {
+            const OboxRec o = t.orr[a];
+            if (!box_overlaps(glo, ghi, o.lo, o.hi)) return kSkip;
+        }
         return leaf ? kLeaf : kOpen;
     }
-    if (MODE == 1) {
-        if (!box_overlaps(olo, ohi, t.mo.lo, t.mo.hi, C, a) && !box_overlaps(glo, ghi, t.olo, t.ohi, C, a))
-            return kSkip;
-        return leaf ? kLeaf : kOpen;
-    }
-    if (t.mo.mm[a] == 0.) return kSkip;
-    const double c[3] = {t.mo.mx[a], t.mo.my[a], t.mo.mz[a]};
+    const MacRec m = t.mr[a];
+    ln = NodeLinks{m.child, m.nchild, m.begin, m.end};
+    if (m.mm == 0.) return kSkip;
+    const double c[3] = {m.mx, m.my, m.mz};
     double s = 0.;
     for (int d = 0; d < 3; d++) {
         double e = 0.;
@@ -486,9 +548,9 @@ __device__ __forceinline__ Visit visit(const TreeView &t, size_t C, int a, const
             e = c[d] - ghi[d];
         s += e * e;
     }
-    const double L = t.nd.len[a];
+    const double L = m.len;
     if (s > L * L * inv_t2) return kSpj;
-    return leaf ? kLeaf : kOpen;
+    return m.child < 0 ? kLeaf : kOpen;
 }
 
 // The host builder's depth-first walk of one group, split at the frontier:
@@ -537,12 +599,13 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
     int na = 0;
     for (int a = t.nd.parent[f]; a >= 0; a = t.nd.parent[a]) anc[na++] = a;
     bool reach = true;
+    NodeLinks ln;
     for (int q = na - 1; q >= 0 && reach; q--) {
         const int a = anc[q];
-        const Visit v = visit<MODE>(t, C, a, olo, ohi, glo, ghi, inv_t2);
+        const Visit v = visit<MODE>(t, a, olo, ohi, glo, ghi, inv_t2, ln);
         if (v == kOpen) continue;
         reach = false;
-        if (v == kSpj && t.nd.begin[a] == t.nd.begin[f]) {
+        if (v == kSpj && ln.begin == t.nd.begin[f]) {
             if (FILL) sout[s0 + ns] = a;
             ns++;
         }
@@ -553,7 +616,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
         st[sp++] = f;
         while (sp > 0) {
             const int a = st[--sp];
-            const Visit v = visit<MODE>(t, C, a, olo, ohi, glo, ghi, inv_t2);
+            const Visit v = visit<MODE>(t, a, olo, ohi, glo, ghi, inv_t2, ln);
             if (v == kSkip) continue;
             if (v == kSpj) {
                 if (FILL) sout[s0 + ns] = a;
@@ -561,7 +624,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
                 continue;
             }
             if (v == kOpen) {
-                const int c0 = t.nd.child[a], nc = t.nd.nchild[a];
+                const int c0 = ln.child, nc = ln.nchild;
                 if (sp + nc > kStack) {   // cannot happen for depth <= 21; stop cleanly
                     atomicOr(err, 1);
                     break;
@@ -575,7 +638,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
                 continue;
             }
             // leaf: its particles, filtered by exact membership for the short lists
-            for (int j = t.nd.begin[a]; j < t.nd.end[a]; j++) {
+            for (int j = ln.begin; j < ln.end; j++) {
                 if (MODE == 2) {
                     if (FILL) jout[j0 + nj] = j;
                     nj++;
@@ -757,6 +820,9 @@ struct sphk_dtree {
     DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx;
     int n_walk = 0, n_front = 0;
     DArr<double> wbox;
+    DArr<BoxRec> brec;
+    DArr<MacRec> mrec;
+    DArr<OboxRec> orec;
     DArr<int64_t> i64a, i64b, i64c, i64d;
     DArr<int64_t> total;
     DArr<int> err;
@@ -778,6 +844,9 @@ struct sphk_dtree {
                         &leaf_idx})
             a->release();
         wbox.release();
+        brec.release();
+        mrec.release();
+        orec.release();
         for (auto *a : {&i64a, &i64b, &i64c, &i64d, &total}) a->release();
         err.release();
         scratch.release();
@@ -848,7 +917,8 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
                            t->y.p, t->z.p, r, t->wbox.p);
     }
     SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
-    TreeView v{t->nodes(), t->moments(), t->olo.p, t->ohi.p, t->x.p, t->y.p, t->z.p, t->group.p, t->i_off.p};
+    TreeView v{t->nodes(), t->moments(), t->brec.p, t->mrec.p, t->orec.p, t->x.p, t->y.p, t->z.p, t->group.p,
+               t->i_off.p};
     const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
     hipLaunchKernelGGL((list_kernel<MODE, false>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
                        t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
@@ -903,7 +973,8 @@ int build_list_two_stage(sphk_dtree *t, const double *r)
     hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p, t->y.p,
                        t->z.p, r, t->wbox.p);
     SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
-    TreeView v{t->nodes(), t->moments(), t->olo.p, t->ohi.p, t->x.p, t->y.p, t->z.p, t->group.p, t->i_off.p};
+    TreeView v{t->nodes(), t->moments(), t->brec.p, t->mrec.p, t->orec.p, t->x.p, t->y.p, t->z.p, t->group.p,
+               t->i_off.p};
     const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
     hipLaunchKernelGGL((list_kernel<MODE, false, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
                        t->i64a.p, nullptr, nullptr, nullptr, nullptr, nullptr, t->err.p);
@@ -1066,6 +1137,10 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
                            t->x.p, t->y.p, t->z.p, m, eps2);
     }
     if (hipGetLastError() != hipSuccess) return fail(set_error(SPHK_E_DEVICE, "dtree: moments launch failed"));
+    if (t->brec.reserve(nn, false, st) != hipSuccess || t->mrec.reserve(nn, false, st) != hipSuccess)
+        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    hipLaunchKernelGGL(pack_nodes_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, st, (int)nn, t->nodes(),
+                       t->moments(), t->brec.p, t->mrec.p);
 
     // groups
     if (t->head.reserve(n, false, st) != hipSuccess || t->i64a.reserve(n, false, st) != hipSuccess ||
@@ -1191,6 +1266,10 @@ int sphk_dtree_list_symmetry(sphk_dtree *t, const double *h)
         hipLaunchKernelGGL(obox_kernel, dim3(grid(na)), dim3(kBlock), 0, t->st, a0, na, t->nodes(), t->olo.p,
                            t->ohi.p, C, t->x.p, t->y.p, t->z.p, h);
     }
+    if (t->orec.reserve(nn, false, t->st) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "dtree symmetry: allocation failed");
+    hipLaunchKernelGGL(pack_obox_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, t->st, (int)nn, t->olo.p, t->ohi.p, C,
+                       t->orec.p);
     SPHK_TRY(hipGetLastError());
     return build_list<1>(t, h, 0.);
 }
