@@ -171,6 +171,8 @@ class DeviceTree:
               "sphk_dtree_build")
         self._h = h
         self._read_walks()
+        self._src = tuple(None if a is None else a.clone() for a in (x, y, z, m, eps2)) + (int(n_leaf),
+                                                                                          int(n_group))
 
     def _read_walks(self):
         import torch
@@ -181,13 +183,23 @@ class DeviceTree:
 
     def rebuild(self, x, y, z, m, eps2=None, n_leaf: int = 8, n_group: int = 64):
         """Build again for new particles into this handle's device buffers
-        (no allocation once they have reached the run's sizes)."""
+        (no allocation once they have reached the run's sizes).  Inputs equal
+        to the last build's (kept as copies) leave the tree as it is."""
+        import torch
         if not self._h:
             raise RuntimeError("tree is closed")
+        src = (x, y, z, m, eps2, int(n_leaf), int(n_group))
+        last = getattr(self, "_src", None)
+        if last is not None and last[5:] == src[5:] and all(
+                (a is None and b is None) or (a is not None and b is not None and a.shape == b.shape
+                                              and torch.equal(a, b)) for a, b in zip(last[:5], src[:5])):
+            return self
         self.n = x.numel()
+        self._src = None
         check(lib().sphk_dtree_rebuild(self._h, self.n, _dptr(x), _dptr(y), _dptr(z), _dptr(m), _dptr(eps2),
                                        int(n_leaf), int(n_group), _stream()), "sphk_dtree_rebuild")
         self._read_walks()
+        self._src = tuple(None if a is None else a.clone() for a in src[:5]) + src[5:]
         return self
 
     def close(self):
