@@ -249,6 +249,21 @@ def main() -> int:
         "itr_after_step": int(sysm.itr.sum().item()),
     }
     out.update(res)
+    # beside the force step: full KDK steps of the same system (predict,
+    # Morton reorder, density retry loop with its gather lists, symmetry and
+    # long lists, the three functors, correct, power-of-two timestep); this
+    # moves the system on, so it runs after every other measurement
+    restore()
+    sysm.force(MIXED)
+    sysm.kdk()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    n_kdk = 3
+    for _ in range(n_kdk):
+        sysm.kdk()
+    torch.cuda.synchronize()
+    out["kdk_ms_per_step"] = round((time.perf_counter() - t) / n_kdk * 1e3, 2)
+    log(f"full KDK step {out['kdk_ms_per_step']} ms")
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

@@ -679,11 +679,11 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
 // else counted into cnt[g].
 constexpr int kMemberCap = 256;   // i-particles of a walk held in LDS
 
-template <int MODE, bool FILL>
+template <int MODE>
 __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const int32_t *__restrict__ lo,
                                                     const int32_t *__restrict__ li, const double *__restrict__ wb,
                                                     const double *__restrict__ r, int64_t *__restrict__ cnt,
-                                                    const int32_t *__restrict__ off, int32_t *__restrict__ jout)
+                                                    const int64_t *__restrict__ off, int32_t *__restrict__ jout)
 {
 #pragma clang fp contract(off)
     __shared__ double si[4][kMemberCap];
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const in
         ghi[d] = t.mo.hi[d * C + gn];
     }
     int64_t count = 0;
-    const int64_t base = FILL ? (int64_t)off[g] : 0;
+    const int64_t base = off[g];
     const int lb = lo[g], le = lo[g + 1];
     for (int l0 = lb; l0 < le; l0 += 64) {
         const int nl = min(64, le - l0);
@@ -768,11 +768,40 @@ __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const in
                     }
             }
             const uint64_t bal = __ballot(pass);
-            if (FILL && pass) jout[base + count + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+            if (pass) jout[base + count + __popcll(bal & ((1ull << lane) - 1ull))] = j;
             count += __popcll(bal);
         }
     }
-    if (!FILL && lane == 0) cnt[g] = count;
+    if (lane == 0) cnt[g] = count;
+}
+
+// Particles in each walk's candidate leaves (the room member_kernel may
+// need for the walk's accepted particles).
+__global__ __launch_bounds__(kBlock) void cand_total_kernel(int ng, const int32_t *__restrict__ lo,
+                                                            const int32_t *__restrict__ li, NodeArrays nd,
+                                                            int64_t *__restrict__ tot)
+{
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ng) return;
+    int64_t s = 0;
+    for (int l = lo[g]; l < lo[g + 1]; l++) {
+        const int a = li[l];
+        s += nd.end[a] - nd.begin[a];
+    }
+    tot[g] = s;
+}
+
+// Accepted particles of each walk, from its stretch of the member scratch
+// to its place in the list.
+__global__ __launch_bounds__(64) void member_copy_kernel(int ng, const int64_t *__restrict__ soff,
+                                                         const int32_t *__restrict__ j_off,
+                                                         const int32_t *__restrict__ scr, int32_t *__restrict__ jout)
+{
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int64_t s0 = soff[g];
+    const int b = j_off[g], n = j_off[g + 1] - b;
+    for (int k = threadIdx.x; k < n; k += 64) jout[b + k] = scr[s0 + k];
 }
 
 // Frontier nodes: the nodes of level L and the leaves above it.
@@ -817,7 +846,7 @@ struct sphk_dtree {
     std::vector<int> level_start;   // node index of the first node of each level
     int n_node = 0;
     DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
-    DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx;
+    DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx, mscr;
     int n_walk = 0, n_front = 0;
     DArr<double> wbox;
     DArr<BoxRec> brec;
@@ -841,7 +870,7 @@ struct sphk_dtree {
             a->release();
         key.release();
         for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &front, &leaf_off,
-                        &leaf_idx})
+                        &leaf_idx, &mscr})
             a->release();
         wbox.release();
         brec.release();
@@ -989,21 +1018,35 @@ int build_list_two_stage(sphk_dtree *t, const double *r)
     hipLaunchKernelGGL((list_kernel<MODE, true, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
                        nullptr, nullptr, t->i64c.p, nullptr, t->leaf_idx.p, nullptr, t->err.p);
     SPHK_TRY(hipGetLastError());
-    // stage 2: exact membership, one wave per walk
+    // stage 2: exact membership, one wave per walk, in one pass: the accepted
+    // particles go to the walk's stretch of a scratch array (room for all its
+    // candidates), then are copied to their offsets in the list
     SPHK_TRY(t->i64b.reserve(ng, false, t->st));
-    SPHK_TRY(t->i64d.reserve(ng, false, t->st));
-    hipLaunchKernelGGL((member_kernel<MODE, false>), dim3(ng), dim3(64), 0, t->st, ng, v, t->leaf_off.p,
-                       t->leaf_idx.p, t->wbox.p, r, t->i64b.p, nullptr, nullptr);
-    SPHK_TRY(hipGetLastError());
+    SPHK_TRY(t->i64d.reserve(ng + 1, false, t->st));
+    hipLaunchKernelGGL(cand_total_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->leaf_off.p,
+                       t->leaf_idx.p, t->nodes(), t->i64b.p);
     if (int rc = scan64(t, t->i64b.p, t->i64d.p, ng)) return rc;
-    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, 1, t->i64d.p, t->i64b.p,
+    int64_t n_cand = 0, last[2];
+    if (hipMemcpyAsync(&last[0], t->i64d.p + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, t->st) != hipSuccess ||
+        hipMemcpyAsync(&last[1], t->i64b.p + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, t->st) != hipSuccess ||
+        hipStreamSynchronize(t->st) != hipSuccess)
+        return set_error(SPHK_E_DEVICE, "dtree: candidate count read failed");
+    n_cand = last[0] + last[1];
+    SPHK_TRY(t->mscr.reserve(std::max<int64_t>(n_cand, 1), false, t->st));
+    hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, t->leaf_off.p, t->leaf_idx.p,
+                       t->wbox.p, r, t->i64b.p, t->i64d.p, t->mscr.p);
+    SPHK_TRY(hipGetLastError());
+    // i64b now holds the accepted counts; their scan gives the list offsets
+    SPHK_TRY(t->i64c.reserve(ng, false, t->st));
+    if (int rc = scan64(t, t->i64b.p, t->i64c.p, ng)) return rc;
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, 1, t->i64c.p, t->i64b.p,
                        L.j_off.p, t->total.p);
     int64_t tot = 0;
     if (int rc = read1(t->st, t->total.p, &tot)) return rc;
     if (tot > std::numeric_limits<int32_t>::max()) return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
     SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot, 1), false, t->st));
-    hipLaunchKernelGGL((member_kernel<MODE, true>), dim3(ng), dim3(64), 0, t->st, ng, v, t->leaf_off.p, t->leaf_idx.p,
-                       t->wbox.p, r, nullptr, L.j_off.p, L.j_idx.p);
+    hipLaunchKernelGGL(member_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->i64d.p, L.j_off.p, t->mscr.p,
+                       L.j_idx.p);
     SPHK_TRY(hipGetLastError());
     int e = 0;
     if (int rc = read1(t->st, t->err.p, &e)) return rc;
