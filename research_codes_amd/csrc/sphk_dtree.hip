@@ -132,15 +132,37 @@ __global__ __launch_bounds__(kBlock) void bbox_partial_kernel(int n, const doubl
 
 // frame = {origin x, y, z, side}: the bounding cube, side enlarged by 1e-9
 // (the host's frame_of).
-__global__ void bbox_final_kernel(int nb, const double *__restrict__ part, double *__restrict__ frame)
+// One block combines the partial boxes (min and max are exact: any order
+// gives the same box).
+__global__ __launch_bounds__(kBlock) void bbox_final_kernel(int nb, const double *__restrict__ part,
+                                                            double *__restrict__ frame)
 {
+    __shared__ double s[6][kBlock];
     const double inf = __builtin_inf();
     double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
-    for (int b = 0; b < nb; b++)
+    for (int b = threadIdx.x; b < nb; b += kBlock)
         for (int d = 0; d < 3; d++) {
             lo[d] = fmin(lo[d], part[6 * b + d]);
             hi[d] = fmax(hi[d], part[6 * b + 3 + d]);
         }
+    for (int d = 0; d < 3; d++) {
+        s[d][threadIdx.x] = lo[d];
+        s[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int d = 0; d < 3; d++) {
+                s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int d = 0; d < 3; d++) {
+        lo[d] = s[d][0];
+        hi[d] = s[3 + d][0];
+    }
     double side = 0.;
     for (int d = 0; d < 3; d++) side = fmax(side, hi[d] - lo[d]);
     if (!(side > 0.)) side = 1.;
@@ -189,7 +211,7 @@ int make_keys(int n, const double *x, const double *y, const double *z, hipStrea
     SPHK_TRY(frame.reserve(4, false, st));
     SPHK_TRY(key.reserve(n, false, st));
     hipLaunchKernelGGL(bbox_partial_kernel, dim3(nb), dim3(kBlock), 0, st, n, x, y, z, part.p);
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(1), 0, st, nb, part.p, frame.p);
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kBlock), 0, st, nb, part.p, frame.p);
     hipLaunchKernelGGL(keys_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, x, y, z,
                        frame.p, key.p, idx);
     SPHK_TRY(hipGetLastError());
@@ -776,19 +798,20 @@ __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const in
 }
 
 // Particles in each walk's candidate leaves (the room member_kernel may
-// need for the walk's accepted particles).
-__global__ __launch_bounds__(kBlock) void cand_total_kernel(int ng, const int32_t *__restrict__ lo,
-                                                            const int32_t *__restrict__ li, NodeArrays nd,
-                                                            int64_t *__restrict__ tot)
+// need for the walk's accepted particles), one wave per walk.
+__global__ __launch_bounds__(64) void cand_total_kernel(int ng, const int32_t *__restrict__ lo,
+                                                        const int32_t *__restrict__ li, NodeArrays nd,
+                                                        int64_t *__restrict__ tot)
 {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
+    const int g = blockIdx.x;
     if (g >= ng) return;
-    int64_t s = 0;
-    for (int l = lo[g]; l < lo[g + 1]; l++) {
+    int s = 0;
+    for (int l = lo[g] + (int)threadIdx.x; l < lo[g + 1]; l += 64) {
         const int a = li[l];
         s += nd.end[a] - nd.begin[a];
     }
-    tot[g] = s;
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (threadIdx.x == 0) tot[g] = s;
 }
 
 // Accepted particles of each walk, from its stretch of the member scratch
@@ -1023,8 +1046,8 @@ int build_list_two_stage(sphk_dtree *t, const double *r)
     // candidates), then are copied to their offsets in the list
     SPHK_TRY(t->i64b.reserve(ng, false, t->st));
     SPHK_TRY(t->i64d.reserve(ng + 1, false, t->st));
-    hipLaunchKernelGGL(cand_total_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->leaf_off.p,
-                       t->leaf_idx.p, t->nodes(), t->i64b.p);
+    hipLaunchKernelGGL(cand_total_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->leaf_off.p, t->leaf_idx.p,
+                       t->nodes(), t->i64b.p);
     if (int rc = scan64(t, t->i64b.p, t->i64d.p, ng)) return rc;
     int64_t n_cand = 0, last[2];
     if (hipMemcpyAsync(&last[0], t->i64d.p + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, t->st) != hipSuccess ||
