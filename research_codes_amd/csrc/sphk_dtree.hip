@@ -40,7 +40,6 @@ using sphk::set_error;
 namespace {
 
 constexpr int kMaxLevel = 21;
-constexpr int kStack = 7 * kMaxLevel + 8;   // depth-first stack bound: 7 per level + 1
 constexpr int kBlock = 256;
 
 #define SPHK_TRY(expr)                                                                          \
@@ -441,19 +440,37 @@ __global__ __launch_bounds__(kBlock) void group_write_kernel(int n, const int32_
 // one 64-byte record per node (one memory segment per visit instead of one
 // per field array).  Values are copies of the node arrays, so the walks'
 // comparisons and their results are unchanged.
+// `next` is the node the depth-first walk visits after a's subtree (its
+// next sibling, else its parent's next, -1 after the last): the walks need
+// no stack.
 struct alignas(64) BoxRec {     // gather and symmetry walks
     double lo[3], hi[3];        // tight box
-    int32_t child, nchild, begin, end;
+    int32_t child, next, begin, end;
 };
 struct alignas(64) MacRec {     // long walk
     double mx, my, mz, mm, len;
-    int32_t child, nchild, begin, end;
+    int32_t child, next, begin, end;
 };
 struct alignas(64) OboxRec {    // symmetry walk: box of the nodes' particle spheres
     double lo[3], hi[3];
 };
 
+// Skip pointers of one level, parents' already set (levels top-down).
+__global__ __launch_bounds__(kBlock) void next_kernel(int a0, int na, NodeArrays nd, int32_t *__restrict__ nxt)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= na) return;
+    const int a = a0 + k;
+    const int p = nd.parent[a];
+    if (p < 0) {
+        nxt[a] = -1;
+        return;
+    }
+    nxt[a] = a < nd.child[p] + nd.nchild[p] - 1 ? a + 1 : nxt[p];
+}
+
 __global__ __launch_bounds__(kBlock) void pack_nodes_kernel(int nn, NodeArrays nd, Moments mo,
+                                                            const int32_t *__restrict__ nxt,
                                                             BoxRec *__restrict__ br, MacRec *__restrict__ mr)
 {
     const int a = blockIdx.x * kBlock + threadIdx.x;
@@ -465,11 +482,11 @@ __global__ __launch_bounds__(kBlock) void pack_nodes_kernel(int nn, NodeArrays n
         b.hi[d] = mo.hi[d * C + a];
     }
     b.child = nd.child[a];
-    b.nchild = nd.nchild[a];
+    b.next = nxt[a];
     b.begin = nd.begin[a];
     b.end = nd.end[a];
     br[a] = b;
-    mr[a] = MacRec{mo.mx[a], mo.my[a], mo.mz[a], mo.mm[a], nd.len[a], b.child, b.nchild, b.begin, b.end};
+    mr[a] = MacRec{mo.mx[a], mo.my[a], mo.mz[a], mo.mm[a], nd.len[a], b.child, b.next, b.begin, b.end};
 }
 
 __global__ __launch_bounds__(kBlock) void pack_obox_kernel(int nn, const double *__restrict__ olo,
@@ -534,7 +551,7 @@ enum Visit { kSkip, kSpj, kLeaf, kOpen };
 
 // Child range and particle range of the visited node, from its record.
 struct NodeLinks {
-    int32_t child, nchild, begin, end;
+    int32_t child, next, begin, end;
 };
 
 template <int MODE>
@@ -545,7 +562,7 @@ __device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&
 #pragma clang fp contract(off)
     if (MODE != 2) {
         const BoxRec b = t.br[a];
-        ln = NodeLinks{b.child, b.nchild, b.begin, b.end};
+        ln = NodeLinks{b.child, b.next, b.begin, b.end};
         const bool leaf = b.child < 0;
         if (MODE == 0) {
             if (!box_overlaps(olo, ohi, b.lo, b.hi)) return kSkip;
@@ -558,7 +575,7 @@ __device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&
         return leaf ? kLeaf : kOpen;
     }
     const MacRec m = t.mr[a];
-    ln = NodeLinks{m.child, m.nchild, m.begin, m.end};
+    ln = NodeLinks{m.child, m.next, m.begin, m.end};
     if (m.mm == 0.) return kSkip;
     const double c[3] = {m.mx, m.my, m.mz};
     double s = 0.;
@@ -616,14 +633,13 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
     const int64_t j0 = FILL ? oj[tid] : 0;
     const int64_t s0 = (FILL && MODE == 2) ? os[tid] : 0;
 
-    // ancestors of f, root first
-    int32_t anc[kMaxLevel + 1];
-    int na = 0;
-    for (int a = t.nd.parent[f]; a >= 0; a = t.nd.parent[a]) anc[na++] = a;
+    // ancestors of f, root first (the frontier is a few levels deep)
+    const int lf = t.nd.level[f];
     bool reach = true;
     NodeLinks ln;
-    for (int q = na - 1; q >= 0 && reach; q--) {
-        const int a = anc[q];
+    for (int L = 0; L < lf && reach; L++) {
+        int a = f;
+        for (int u = lf; u > L; u--) a = t.nd.parent[a];
         const Visit v = visit<MODE>(t, a, olo, ohi, glo, ghi, inv_t2, ln);
         if (v == kOpen) continue;
         reach = false;
@@ -633,25 +649,21 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
         }
     }
     if (reach) {
-        int32_t st[kStack];
-        int sp = 0;
-        st[sp++] = f;
-        while (sp > 0) {
-            const int a = st[--sp];
+        // preorder over f's subtree by skip pointers: open = first child,
+        // otherwise continue at next (f's own next ends the subtree)
+        const int stop = MODE == 2 ? t.mr[f].next : t.br[f].next;
+        for (int nxt = f; nxt != stop;) {
+            const int a = nxt;
             const Visit v = visit<MODE>(t, a, olo, ohi, glo, ghi, inv_t2, ln);
+            if (v == kOpen) {
+                nxt = ln.child;
+                continue;
+            }
+            nxt = ln.next;
             if (v == kSkip) continue;
             if (v == kSpj) {
                 if (FILL) sout[s0 + ns] = a;
                 ns++;
-                continue;
-            }
-            if (v == kOpen) {
-                const int c0 = ln.child, nc = ln.nchild;
-                if (sp + nc > kStack) {   // cannot happen for depth <= 21; stop cleanly
-                    atomicOr(err, 1);
-                    break;
-                }
-                for (int cc = c0 + nc - 1; cc >= c0; cc--) st[sp++] = cc;
                 continue;
             }
             if (LEAVES) {   // candidate leaf; its particles are filtered by member_kernel
@@ -697,8 +709,8 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
 // c0 + l of the walk's leaves laid end to end (the host's order), and a
 // ballot orders the accepted ones.  MODE 0: |x_j - x_i| < rs_i for some i;
 // MODE 1: r_ij < max(H_i, H_j) for some i (after the same box pre-tests as
-// the host's leaf loop).  With FILL the accepted j are written from off[g],
-// else counted into cnt[g].
+// the host's leaf loop).  The accepted j are written from off[g] (room for
+// all of the walk's candidates) and their number to cnt[g].
 constexpr int kMemberCap = 256;   // i-particles of a walk held in LDS
 
 template <int MODE>
@@ -869,7 +881,7 @@ struct sphk_dtree {
     std::vector<int> level_start;   // node index of the first node of each level
     int n_node = 0;
     DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
-    DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx, mscr;
+    DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx, mscr, nextp;
     int n_walk = 0, n_front = 0;
     DArr<double> wbox;
     DArr<BoxRec> brec;
@@ -893,7 +905,7 @@ struct sphk_dtree {
             a->release();
         key.release();
         for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &front, &leaf_off,
-                        &leaf_idx, &mscr})
+                        &leaf_idx, &mscr, &nextp})
             a->release();
         wbox.release();
         brec.release();
@@ -1203,10 +1215,15 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
                            t->x.p, t->y.p, t->z.p, m, eps2);
     }
     if (hipGetLastError() != hipSuccess) return fail(set_error(SPHK_E_DEVICE, "dtree: moments launch failed"));
-    if (t->brec.reserve(nn, false, st) != hipSuccess || t->mrec.reserve(nn, false, st) != hipSuccess)
+    if (t->brec.reserve(nn, false, st) != hipSuccess || t->mrec.reserve(nn, false, st) != hipSuccess ||
+        t->nextp.reserve(nn, false, st) != hipSuccess)
         return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
+    for (int L = 0; L + 1 < (int)t->level_start.size(); L++) {
+        const int a0 = t->level_start[L], na = t->level_start[L + 1] - a0;
+        hipLaunchKernelGGL(next_kernel, dim3(grid(na)), dim3(kBlock), 0, st, a0, na, t->nodes(), t->nextp.p);
+    }
     hipLaunchKernelGGL(pack_nodes_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, st, (int)nn, t->nodes(),
-                       t->moments(), t->brec.p, t->mrec.p);
+                       t->moments(), t->nextp.p, t->brec.p, t->mrec.p);
 
     // groups
     if (t->head.reserve(n, false, st) != hipSuccess || t->i64a.reserve(n, false, st) != hipSuccess ||
