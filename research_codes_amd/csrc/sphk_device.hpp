@@ -91,17 +91,8 @@ __device__ __forceinline__ R kernel0th(R q)
     return R(kCeff0) * rmin2 * rmin2 * (R(1) + R(4) * q);
 }
 
-// W'(q) = C1 [(1-q)_+^4 - (1-q)_+^3 (1+4q)], hdr_kernel.hpp:251-257.
-template <typename R>
-__device__ __forceinline__ R kernel1st(R q)
-{
-    R rmin = R(1) - q;
-    rmin = rmin > R(0) ? rmin : R(0);
-    const R rmin2 = rmin * rmin;
-    const R rmin3 = rmin * rmin2;
-    const R rmin4 = rmin2 * rmin2;
-    return R(kCeff1) * (rmin4 - rmin3 * (R(1) + R(4) * q));
-}
+// W'(q) = C1 [(1-q)_+^4 - (1-q)_+^3 (1+4q)] (hdr_kernel.hpp:251-257) equals
+// -5 C1 q (1-q)_+^3; the pair terms use that closed form directly.
 
 // Butterfly sum over the lanes that share one i-particle.  Lane l holds i
 // (l & (P-1)) and j-slice (l >> logP); masks P..32 combine the slices.  Every
