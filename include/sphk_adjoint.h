@@ -46,6 +46,27 @@ int sphk_gravity_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const 
                          const double *ebar, double *gx, double *gy, double *gz, double *nx, double *ny,
                          double *nz, void *stream);
 
+/* Cotangents of the hydro functor's differentiable inputs (sphk_hydro_src
+ * fields other than id and m), accumulated by sphk_hydro_vjp_dev. */
+typedef struct {
+    double *x, *y, *z, *vx, *vy, *vz;
+    double *hinv, *pres, *presc, *bswt, *dens, *vsnd, *alph, *alphu, *thrm, *eta;
+} sphk_hydro_grad;
+
+/* Vector-Jacobian product of the FP64 hydro functor (sphk_hydro_dev,
+ * hdr_hydro.hpp:320-421) over symmetric lists (each walk's list holds every
+ * j with r_ij < max(H_i, H_j) for some i of the walk, as the symmetry list
+ * builders make them): given cotangents of the outputs ax, ay, az, udot,
+ * diffu, gx, gy, gz (vsmx is not differentiated; its pointer is ignored),
+ * adds to grad the cotangents of every particle's inputs.  One set of
+ * particle arrays serves as both the i- and the j-side.  Each particle's
+ * cotangent is summed by its own lanes in a fixed order: no atomics,
+ * bitwise reproducible.  This operation has no counterpart in the
+ * reference; it serves callers that differentiate collision outcomes. */
+int sphk_hydro_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
+                       const int32_t *j_idx, sphk_hydro_src src, sphk_hydro_out cot,
+                       sphk_hydro_grad grad, void *stream);
+
 /* gx[k] += m_k * sum over the nodes n containing k of nx[n] / M_n (same for
  * y, z); m = the particle masses the tree's monopoles were built from. */
 int sphk_dtree_spread_dev(const sphk_dtree *t, const double *m, const double *nx, const double *ny,

@@ -37,6 +37,8 @@ DENS_OUT = ("dens", "divv", "rotv", "grdh", "ksr", "np", "itr")
 HYDRO_SRC = ("id", "m", "x", "y", "z", "vx", "vy", "vz", "hinv", "pres", "presc", "bswt",
              "dens", "vsnd", "alph", "alphu", "thrm", "eta")
 HYDRO_OUT = ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx", "diffu")
+HYDRO_GRAD = ("x", "y", "z", "vx", "vy", "vz", "hinv", "pres", "presc", "bswt", "dens", "vsnd", "alph",
+              "alphu", "thrm", "eta")
 GRAV_SRC = ("m", "x", "y", "z", "eps2")
 GRAV_OUT = ("ax", "ay", "az", "pot", "eta")
 
@@ -49,6 +51,7 @@ DensSrc = _struct("DensSrc", DENS_SRC)
 DensOut = _struct("DensOut", DENS_OUT)
 HydroSrc = _struct("HydroSrc", HYDRO_SRC)
 HydroOut = _struct("HydroOut", HYDRO_OUT)
+HydroGrad = _struct("HydroGrad", HYDRO_GRAD)
 GravSrc = _struct("GravSrc", GRAV_SRC)
 GravOut = _struct("GravOut", GRAV_OUT)
 
@@ -137,6 +140,7 @@ SIGNATURES.update({
     "sphk_gravity_vjp_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, GravSrc, _P, _P, GravSrc, GravSrc, C.c_double]
                              + [_P] * 11 + [_P]),
     "sphk_dtree_spread_dev": (C.c_int, [_P] + [_P] * 7),
+    "sphk_hydro_vjp_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, HydroSrc, HydroOut, HydroGrad, _P]),
 })
 
 _lib = None

@@ -161,6 +161,17 @@ class Context:
                                    _mk(L.HydroOut, L.HYDRO_OUT, out), int(precision),
                                    self._stream()), "sphk_hydro_dev")
 
+    def hydro_vjp(self, w: Walks, src: dict, cot: dict, grad: dict):
+        """Add to grad (HYDRO_GRAD fields) the vector-Jacobian product of the
+        FP64 hydro functor over symmetric lists w, for output cotangents cot
+        (HYDRO_OUT fields; vsmx unused)."""
+        check(lib().sphk_hydro_vjp_dev(self._h, w.n_walk, w.i_off.data_ptr(), w.j_off.data_ptr(),
+                                       None if w.j_idx is None else w.j_idx.data_ptr(),
+                                       _mk(L.HydroSrc, L.HYDRO_SRC, src),
+                                       _mk(L.HydroOut, L.HYDRO_OUT, {k: v for k, v in cot.items() if k != "vsmx"}),
+                                       _mk(L.HydroGrad, L.HYDRO_GRAD, grad), self._stream()),
+              "sphk_hydro_vjp_dev")
+
     def gravity(self, w: Walks, pj: dict, pi: dict, out: dict, ps: dict | None = None,
                 precision: int = FP64):
         check(lib().sphk_gravity_dev(self._h, w.n_walk, w.i_off.data_ptr(), w.j_off.data_ptr(),

@@ -16,9 +16,10 @@ The restatement follows the functors and glue the HIP step runs:
 * EOS and Balsara switch (hdr_hgas.hpp:318-344, hdr_sph.hpp:79-89);
 * tree gravity and its write-back (hdr_gravity.hpp:107-113, 295-338),
   through the hand-written vector-Jacobian product (adjoint.TreeGravity);
-* the hydro functor (hdr_hydro.hpp:320-421), its EPI preparation
-  (hdr_hydro.hpp:50-60) and write-back with the acceleration sum and the
-  viscosity switches' rates (hdr_hydro.hpp:21-25, hdr_sph.hpp:95-97,
+* the hydro functor (hdr_hydro.hpp:320-421) on the device, through its
+  hand-written vector-Jacobian product (sphk_hydro_vjp_dev), its EPI
+  preparation (hdr_hydro.hpp:50-60) and write-back with the acceleration sum
+  and the viscosity switches' rates (hdr_hydro.hpp:21-25, hdr_sph.hpp:95-97,
   hdr_hgas.hpp:750-762).
 
 As in any tree code's derivative, the tree, the lists and the reorder are
@@ -36,6 +37,7 @@ import math
 import numpy as np
 import torch
 
+from . import _lib as L
 from . import ic
 from .adjoint import TreeGravity
 from .device import FP64, Context, Walks
@@ -212,7 +214,8 @@ def _density_radius(P, m, pairs, hmax, eta, ksrh):
 
 
 def _hydro_t(P, m, pairs, d, G):
-    """The hydro functor on non-self pairs (i, j) with r < max(H_i, H_j)."""
+    """The hydro functor on non-self pairs (i, j) with r < max(H_i, H_j), in
+    torch (the autograd reference the device VJP is tested against)."""
     i, j = pairs
     seg = _Seg(i, j, m.numel())
     gi, gj = seg.gi, seg.gj
@@ -245,6 +248,41 @@ def _hydro_t(P, m, pairs, d, G):
     return out
 
 
+_HYDRO_OUTS = ("ax", "ay", "az", "udot", "diffu", "gx", "gy", "gz")
+
+
+class _HydroFn(torch.autograd.Function):
+    """The FP64 hydro functor on the device (sphk_hydro_dev) over symmetric
+    lists, differentiated by its hand-written vector-Jacobian product
+    (sphk_hydro_vjp_dev).  Inputs: the HYDRO_GRAD fields in that order."""
+
+    @staticmethod
+    def forward(ctx, walks, dctx, m, ids, *vals):
+        src = dict(zip(L.HYDRO_GRAD, (v.contiguous() for v in vals)))
+        src.update(id=ids, m=m)
+        out = {k: torch.empty_like(m) for k in L.HYDRO_OUT}
+        dctx.hydro(walks, src, src, out, FP64)
+        ctx.save_for_backward(*(src[k] for k in L.HYDRO_GRAD))
+        ctx.walks, ctx.dctx, ctx.m, ctx.ids = walks, dctx, m, ids
+        return tuple(out[k] for k in _HYDRO_OUTS)
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        src = dict(zip(L.HYDRO_GRAD, ctx.saved_tensors))
+        src.update(id=ctx.ids, m=ctx.m)
+        cot = {k: torch.zeros_like(ctx.m) if g is None else g.contiguous() for k, g in zip(_HYDRO_OUTS, gouts)}
+        grad = {k: torch.zeros_like(ctx.m) for k in L.HYDRO_GRAD}
+        ctx.dctx.hydro_vjp(ctx.walks, src, cot, grad)
+        return (None, None, None, None) + tuple(grad[k] for k in L.HYDRO_GRAD)
+
+
+def hydro_dev(walks, dctx, m, ids, inputs: dict) -> dict:
+    """Differentiable hydro functor outputs (ax, ay, az, udot, diffu, gx,
+    gy, gz; g without the gravitational constant) from the HYDRO_GRAD inputs."""
+    out = _HydroFn.apply(walks, dctx, m.detach(), ids, *(inputs[k] for k in L.HYDRO_GRAD))
+    return dict(zip(_HYDRO_OUTS, out))
+
+
 def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
     """One force evaluation (density, EOS + Balsara, gravity + write-back,
     hydro + write-back + rates) on the recorded lists, differentiable in
@@ -269,21 +307,18 @@ def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
                      n_leaf=sysm.n_leaf, precision=FP64, G=ph.G)
     g2x, g2y, g2z, pot, etag = tg.with_eta(P["x"], P["y"], P["z"])
     eta = H * H / (ksrh * ksrh) * omg / (3.0 * dens) * etag
-    # hydro
-    d = {"hinv": 1.0 / H, "presc": pres * omg / (dens * dens), "bswt_half": 0.5 * bswt,
-         "thrm": torch.clamp(u, min=0.0), "vsnd": vsnd, "dens": dens, "pres": pres, "eta": eta}
-    i, j = expand_pairs(rec["hydro"])
-    with torch.no_grad():
-        r2 = sum((P[c][i] - P[c][j]) ** 2 for c in ("x", "y", "z"))
-        hmx = torch.maximum(H[i], H[j])
-        keep = (i != j) & (r2 < hmx * hmx)
-    h = _hydro_t(P, m, (i[keep], j[keep]), d, ph.G)
-    del i, j, r2, hmx, keep
+    # hydro: the device functor and its hand-written VJP, on the EPI
+    # preparation of hdr_hydro.hpp:50-60
+    hin = {"x": P["x"], "y": P["y"], "z": P["z"], "vx": P["vx"], "vy": P["vy"], "vz": P["vz"],
+           "hinv": 1.0 / H, "pres": pres, "presc": pres * omg / (dens * dens), "bswt": 0.5 * bswt,
+           "dens": dens, "vsnd": vsnd, "alph": P["alph"], "alphu": P["alphu"],
+           "thrm": torch.clamp(u, min=0.0), "eta": eta}
+    h = hydro_dev(rec["hydro"], sysm.ctx, m, rec["id"], hin)
     # write-back: acceleration sum and the viscosity switches' rates
     out = {"dens": dens, "ksr": H, "pot": pot, "udot": h["udot"]}
-    out["ax"] = h["achx"] + h["g1x"] + g2x
-    out["ay"] = h["achy"] + h["g1y"] + g2y
-    out["az"] = h["achz"] + h["g1z"] + g2z
+    out["ax"] = h["ax"] + ph.G * h["gx"] + g2x
+    out["ay"] = h["ay"] + ph.G * h["gy"] + g2y
+    out["az"] = h["az"] + ph.G * h["gz"] + g2z
     alph, alphu = P["alph"], P["alphu"]
     src = torch.clamp(-divv * (ph.alpha_max - alph), min=0.0)
     tauinv = 0.25 * ksrh * vsnd / H

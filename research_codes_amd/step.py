@@ -471,14 +471,16 @@ class SPHSystem:
         self.gravity()
         self.hydro()
         if record is not None:
-            record.update(perm=None, hmax=hmax, dens=self.lists["dens"], hydro=self.lists["hydro"])
+            record.update(perm=None, hmax=hmax, dens=self.lists["dens"], hydro=self.lists["hydro"],
+                          id=self.id.clone())
         self.update_hmax()
 
     def kdk(self, dt: float | None = None, record: dict | None = None):
         """One loopSimulation step (hdr_hgas.hpp:1499-1539) with a fixed list
         rebuild per step.  dt overrides the power-of-two timestep rule (a
         fixed schedule, e.g. for finite-difference sensitivities).  `record`
-        (a dict) receives the step's reorder, H_max and lists.  Returns the
+        (a dict) receives the step's reorder, H_max, lists and particle ids
+        (in the reordered order).  Returns the
         diagnostics before the step."""
         d = self.diagnostics()
         self.dt = self.next_dt(d["dt_min"]) if dt is None else float(dt)
@@ -494,7 +496,7 @@ class SPHSystem:
         self.hydro()
         if record is not None:
             record.update(perm=self.last_perm, hmax=hmax, dens=self.lists["dens"],
-                          hydro=self.lists["hydro"])
+                          hydro=self.lists["hydro"], id=self.id.clone())
         self.update_hmax()
         check(lib().sphk_correct_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_correct_dev")
