@@ -126,7 +126,9 @@ class _Seg:
         if i.numel() > 1 and not bool((i[1:] >= i[:-1]).all()):
             raise ValueError("pairs must be sorted by their i index")
         self.i, self.j = i, j
-        self.lengths = torch.bincount(i, minlength=n)
+        # pairs per i from the sorted i (cheaper than a histogram)
+        bounds = torch.searchsorted(i, torch.arange(n + 1, device=i.device, dtype=i.dtype))
+        self.lengths = bounds[1:] - bounds[:-1]
 
     def sum(self, v):
         return _SegSum.apply(v, self.i, self.lengths)
