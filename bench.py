@@ -533,11 +533,81 @@ def cpu_baseline(sysm, entries, seconds, log):
     step_s = sum(entries[k] / (res[k][0] / res[k][1]) for k in res)
     total = sum(entries.values())
     log("cpu baseline: " + ", ".join(f"{k} {res[k][0]:.3g} entries in {res[k][1]:.1f}s" for k in res))
+
+    # the oracle as the checker at full size: the functors' outputs (FP64 and
+    # mixed, all walks, same inputs) against the oracle's on sampled walks
+    import torch
+    from research_codes_amd.device import FP64, MIXED
+    W = sysm.lists
+    gdev = dict(m=sysm.s["m"], x=sysm.s["x"], y=sysm.s["y"], z=sysm.s["z"], eps2=sysm.eps2)
+    gpu = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        od = {k: torch.empty_like(sysm.s["m"]) for k in ("dens", "divv", "rotv", "grdh", "ksr")}
+        od["np"] = torch.empty_like(sysm.np_)
+        od["itr"] = torch.empty_like(sysm.itr)
+        sysm.ctx.density(W["dens"], sysm.dens_src(), sysm.dens_src(), od, prec)
+        oh = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx",
+                                                         "diffu")}
+        sysm.ctx.hydro(W["hydro"], sysm.hydro_src(), sysm.hydro_src(), oh, prec)
+        og = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "pot", "eta")}
+        sysm.ctx.gravity(W["grav"], gdev, gdev, og, ps=sysm.mom, precision=prec)
+        gpu[name] = {"dens": {k: v.cpu().numpy() for k, v in od.items()},
+                     "hydro": {k: v.cpu().numpy() for k, v in oh.items()},
+                     "grav": {k: v.cpu().numpy() for k, v in og.items()}}
+    ref = {"dens": {}, "hydro": {}, "grav": {}}
+    parts = []
+    for kind in ("dens", "hydro", "grav"):
+        i_off, j_off, j_idx, s_off, s_idx = host[kind]
+        nw = len(i_off) - 1
+        for b in range(4):
+            sl = slice(b * nw // 4, b * nw // 4 + 33)
+            io, jo = i_off[sl], j_off[sl]
+            if kind == "dens":
+                o = oracle.density(io, jo, j_idx, dsrc, dsrc, eta=sysm.eta, hmax=sysm.ctx.params["hmax"],
+                                   nthreads=nthreads)
+            elif kind == "hydro":
+                o = oracle.hydro(io, jo, j_idx, hsrc, hsrc, nthreads=nthreads)
+            else:
+                o = oracle.gravity(io, jo, j_idx, gsrc, gsrc, s_off=s_off[sl], s_idx=s_idx, ps=mom,
+                                   nthreads=nthreads)
+            idx = np.arange(int(io[0]), int(io[-1]))
+            for k, v in o.items():
+                ref[kind].setdefault(k, []).append(v[idx])
+            if kind == "dens":
+                parts.append(idx)
+    ref = {kind: {k: np.concatenate(v) for k, v in d.items()} for kind, d in ref.items()}
+
+    def sel(kind):
+        i_off = host[kind][0]
+        nw = len(i_off) - 1
+        return np.concatenate([np.arange(int(i_off[b * nw // 4]), int(i_off[b * nw // 4 + 32]))
+                               for b in range(4)])
+
+    parity = {"walks": 4 * 32, "particles": int(len(np.concatenate(parts)))}
+    for name in gpu:
+        errs, np_mis = {}, 0
+        for kind in ("dens", "hydro", "grav"):
+            ix = sel(kind)
+            for k, r in ref[kind].items():
+                if k == "itr":
+                    continue
+                g = gpu[name][kind][k][ix]
+                if k == "np":
+                    np_mis += int((g != r).sum())
+                    continue
+                # relative to the field's largest value over the sample
+                errs[f"{kind}.{k}"] = float(np.abs(g - r).max() / max(np.abs(r).max(), 1e-300))
+        parity[name] = {"np_mismatch": np_mis, "max_rel_err": max(errs.values()),
+                        "worst_field": max(errs, key=errs.get), "per_field": errs}
+    log(f"parity on sampled walks: fp64 max {parity['fp64']['max_rel_err']:.2e}, mixed max "
+        f"{parity['mixed']['max_rel_err']:.2e}, np mismatches {parity['fp64']['np_mismatch']}"
+        f"/{parity['mixed']['np_mismatch']}")
     return {"value": total / step_s, "unit": "interactions/s", "cores": nthreads, "kind": "port",
             "cpu": cpu_model(), "ms_per_step_est": round(step_s * 1e3, 1),
             "sample": "oracle (scalar FP64 C, OpenMP over walks) on 8 evenly spaced blocks of "
                       "consecutive walks per functor, same lists and inputs as the GPU step; "
-                      + ", ".join(f"{k}: {res[k][0]:.3g} entries / {res[k][1]:.1f}s" for k in res)}
+                      + ", ".join(f"{k}: {res[k][0]:.3g} entries / {res[k][1]:.1f}s" for k in res),
+            "parity_vs_gpu": parity}
 
 
 if __name__ == "__main__":
