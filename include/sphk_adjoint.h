@@ -46,6 +46,30 @@ int sphk_gravity_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const 
                          const double *ebar, double *gx, double *gy, double *gz, double *nx, double *ny,
                          double *nz, void *stream);
 
+/* Cotangents of the density functor's outputs, and of its differentiable
+ * inputs (positions, velocities, the starting smoothing length ksr). */
+typedef struct {
+    const double *dens, *ksr, *divv, *rotv, *grdh;
+} sphk_dens_cot;
+typedef struct {
+    double *x, *y, *z, *vx, *vy, *vz, *ksr;
+} sphk_dens_grad;
+
+/* Vector-Jacobian product of the FP64 density functor (sphk_density_dev,
+ * hdr_density.hpp:254-363: three smoothing-length passes from ksr, then the
+ * grad-h, divergence and curl pass) over gather lists holding every j within
+ * the largest H of the passes (as the functor's retry loop leaves them), with
+ * the context's eta, kappa and H_max (a capped H has zero derivative): given
+ * cotangents of dens, ksr (H_3), divv, rotv and grdh, adds to grad the
+ * cotangents of every particle's x, y, z, vx, vy, vz and of the i-particles'
+ * starting ksr.  np and itr are not differentiated.  One set of particle
+ * arrays serves as both sides.  j-side terms use FP64 atomics (the summation
+ * order is not fixed).  No counterpart in the reference; for callers that
+ * differentiate collision outcomes. */
+int sphk_density_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
+                         const int32_t *j_idx, sphk_dens_src src, sphk_dens_cot cot, sphk_dens_grad grad,
+                         void *stream);
+
 /* Cotangents of the hydro functor's differentiable inputs (sphk_hydro_src
  * fields other than id and m), accumulated by sphk_hydro_vjp_dev. */
 typedef struct {

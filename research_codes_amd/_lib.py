@@ -39,6 +39,8 @@ HYDRO_SRC = ("id", "m", "x", "y", "z", "vx", "vy", "vz", "hinv", "pres", "presc"
 HYDRO_OUT = ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx", "diffu")
 HYDRO_GRAD = ("x", "y", "z", "vx", "vy", "vz", "hinv", "pres", "presc", "bswt", "dens", "vsnd", "alph",
               "alphu", "thrm", "eta")
+DENS_COT = ("dens", "ksr", "divv", "rotv", "grdh")
+DENS_GRAD = ("x", "y", "z", "vx", "vy", "vz", "ksr")
 GRAV_SRC = ("m", "x", "y", "z", "eps2")
 GRAV_OUT = ("ax", "ay", "az", "pot", "eta")
 
@@ -52,6 +54,8 @@ DensOut = _struct("DensOut", DENS_OUT)
 HydroSrc = _struct("HydroSrc", HYDRO_SRC)
 HydroOut = _struct("HydroOut", HYDRO_OUT)
 HydroGrad = _struct("HydroGrad", HYDRO_GRAD)
+DensCot = _struct("DensCot", DENS_COT)
+DensGrad = _struct("DensGrad", DENS_GRAD)
 GravSrc = _struct("GravSrc", GRAV_SRC)
 GravOut = _struct("GravOut", GRAV_OUT)
 
@@ -141,6 +145,7 @@ SIGNATURES.update({
                              + [_P] * 11 + [_P]),
     "sphk_dtree_spread_dev": (C.c_int, [_P] + [_P] * 7),
     "sphk_hydro_vjp_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, HydroSrc, HydroOut, HydroGrad, _P]),
+    "sphk_density_vjp_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, DensSrc, DensCot, DensGrad, _P]),
 })
 
 _lib = None

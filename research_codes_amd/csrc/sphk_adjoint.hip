@@ -365,8 +365,298 @@ __global__ __launch_bounds__(kWave) void hydro_vjp_kernel(int n_walk, const int3
     }
 }
 
+// ---------------------------------------------------------------------------
+// Density functor VJP (FP64), hdr_density.hpp:254-363.  Per i-particle the
+// forward is recomputed: three h-passes rho_k = h_k^3 sum m W(r h_k),
+// H_{k+1} = min(eta kappa (m / rho_k)^(1/3), hmax), with S_k = sum m r^2 t^3
+// (so that d rho_k / d h_k = 3 rho_k / h_k - 5 C1 h_k^4 S_k), then the grad-h
+// sums at H_3.  The reverse runs the outputs' cotangents back through
+// Omega, div v and |curl v| (per particle), through the grad-h pair sums (one
+// sweep), down the chain H_3 -> rho_2 -> H_2 -> rho_1 -> H_1 -> rho_0 -> H_0
+// (per particle), and through the three h-passes' pair sums (one sweep with
+// the three kernel weights).  Gather lists are not symmetric, so the j-side
+// position and velocity cotangents collect in LDS per tile (atomic adds) and
+// leave with one global FP64 atomic per record and component.
+// ---------------------------------------------------------------------------
+struct DRec {
+    double x, y, z, m, vx, vy, vz;
+    int32_t id;
+};
+
+// Per-i sums over the S slices (fixed order), visible to every lane of the i.
+template <int N>
+__device__ __forceinline__ void slice_total(double (&v)[N], double *red, int lane, int il, int ni, int S)
+{
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        __syncthreads();
+        red[q * kWave + lane] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        double s = 0.;
+        for (int t = 0; t < S; t++) s += red[q * kWave + il + ni * t];
+        v[q] = s;
+    }
+}
+
+__global__ __launch_bounds__(kWave) void density_vjp_kernel(int n_walk, const int32_t *__restrict__ i_off,
+                                                            const int32_t *__restrict__ j_off,
+                                                            const int32_t *__restrict__ j_idx, sphk_dens_src src,
+                                                            double eta_ksrh, double hmax, sphk_dens_cot cot,
+                                                            sphk_dens_grad grad)
+{
+    constexpr double C0 = kCeff0, k5 = -5. * kCeff1;
+    __shared__ DRec sp[kTile];
+    __shared__ int32_t sj[kTile];
+    __shared__ double jb[6 * kTile];
+    __shared__ double red[6 * kWave];
+    if ((int)blockIdx.x >= n_walk) return;
+    const int w = xcd_walk_adj(blockIdx.x, n_walk);
+    const int lane = threadIdx.x;
+    const int ib = i_off[w], ie = i_off[w + 1];
+    const int kb = j_off[w], ke = j_off[w + 1];
+
+    // one sweep over the walk's list: f(record, k-th slot) per pair of this
+    // lane's slice; with FLUSH the j-side cotangents in jb[0..5][k] go to grad
+    auto sweep = [&](auto &&f, bool flush, int slice, int S) {
+        for (int t0 = kb; t0 < ke; t0 += kTile) {
+            const int nt = min(kTile, ke - t0);
+            __syncthreads();
+            if (lane < nt) {
+                const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
+                sp[lane] = DRec{src.x[j], src.y[j], src.z[j], src.m[j], src.vx[j], src.vy[j], src.vz[j], src.id[j]};
+                sj[lane] = j;
+            } else {
+                sp[lane] = DRec{1e15, 1e15, 1e15, 0., 0., 0., 0., -1};
+                sj[lane] = -1;
+            }
+            if (flush)
+                for (int q = 0; q < 6; q++) jb[q * kTile + lane] = 0.;
+            __syncthreads();
+            for (int k = slice; k < kTile; k += S) f(sp[k], k);
+            if (flush) {
+                __syncthreads();
+                if (lane < nt) {
+                    const int j = sj[lane];
+                    double *dst[6] = {grad.x, grad.y, grad.z, grad.vx, grad.vy, grad.vz};
+                    for (int q = 0; q < 6; q++) {
+                        const double v = jb[q * kTile + lane];
+                        if (v != 0.) atomicAdd(&dst[q][j], v);
+                    }
+                }
+            }
+        }
+    };
+
+    for (int c0 = ib; c0 < ie; c0 += kWave) {
+        const int ni = min(kWave, ie - c0);
+        const int S = kWave / ni, il = lane % ni;
+        const int slice = lane / ni < S ? lane / ni : kTile;   // idle lanes sweep nothing
+        const bool lead = lane < ni;
+        const int i = c0 + il;
+        const double px = src.x[i], py = src.y[i], pz = src.z[i];
+        const double pvx = src.vx[i], pvy = src.vy[i], pvz = src.vz[i];
+        const double mi = src.m[i];
+
+        // -- forward: three h-passes (rho_k, S_k) and H_{k+1}
+        double H[4], rho[3], Sk[3];
+        bool capped[4] = {false, false, false, false};
+        H[0] = src.ksr[i];
+        for (int k = 0; k < 3; k++) {
+            const double h = 1. / H[k];
+            double v[2] = {0., 0.};
+            sweep([&](const DRec &r, int) {
+                const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double q = __builtin_sqrt(r2) * h;
+                const double t = q < 1. ? 1. - q : 0.;
+                const double t2 = t * t;
+                v[0] += r.m * (C0 * t2 * t2 * (1. + 4. * q));
+                v[1] += r.m * r2 * (t2 * t);
+            }, false, slice, S);
+            slice_total<2>(v, red, lane, il, ni, S);
+            rho[k] = v[0] * (h * h * h);
+            Sk[k] = v[1];
+            const double hn = eta_ksrh * cbrt_f64(mi / rho[k]);
+            capped[k + 1] = !(hn < hmax);
+            H[k + 1] = hn < hmax ? hn : hmax;
+        }
+        // -- forward grad-h sums at H_3
+        const double h3 = 1. / H[3];
+        const double h3_4 = (h3 * h3) * (h3 * h3), h3_5 = h3_4 * h3;
+        double fg[5] = {0., 0., 0., 0., 0.};   // gsum, dvsum, rot x, y, z (dw folded in)
+        sweep([&](const DRec &r, int) {
+            const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+            const double dvx = pvx - r.vx, dvy = pvy - r.vy, dvz = pvz - r.vz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double q = __builtin_sqrt(r2) * h3;
+            const double t = q < 1. ? 1. - q : 0.;
+            const double t3 = t * t * t;
+            const double dw = k5 * r.m * h3_5 * t3;
+            fg[0] += r.m * t3 * (3. + q * (9. - 32. * q));
+            fg[1] += (dvx * dx + dvy * dy + dvz * dz) * dw;
+            fg[2] += (dvy * dz - dvz * dy) * dw;
+            fg[3] += (dvz * dx - dvx * dz) * dw;
+            fg[4] += (dvx * dy - dvy * dx) * dw;
+        }, false, slice, S);
+        slice_total<5>(fg, red, lane, il, ni, S);
+        const double g = -C0 * h3_4 * fg[0];
+        const double dvv = -fg[1];
+        const double rho2 = rho[2];
+        const double B = 1. + H[3] * g / (3. * rho2);
+        const double Om = 1. / B;
+        const double rotm2 = fg[2] * fg[2] + fg[3] * fg[3] + fg[4] * fg[4];
+        const double rotm = rotm2 > 0. ? __builtin_sqrt(rotm2) : 0.;
+
+        // -- reverse through the outputs (per particle)
+        const double cD = cot.dens[i], cK = cot.ksr[i], cV = cot.divv[i], cR = cot.rotv[i], cG = cot.grdh[i];
+        const double Omb = cG + (cV * dvv + cR * rotm) / rho2;
+        double rho2b = cD - (cV * dvv + cR * rotm) * Om / (rho2 * rho2);
+        const double A = Om / rho2;
+        const double dvsb = -cV * A;   // d L / d dvsum (dv = -dvsum)
+        double rb[3] = {0., 0., 0.};
+        if (rotm > 0.)
+            for (int q = 0; q < 3; q++) rb[q] = cR * A * fg[2 + q] / rotm;
+        const double Bb = -Omb * Om * Om;
+        double H3b = cK + Bb * g / (3. * rho2);
+        const double gb = Bb * H[3] / (3. * rho2);
+        rho2b -= Bb * H[3] * g / (3. * rho2 * rho2);
+        const double gsb = -C0 * h3_4 * gb;            // d L / d gsum
+        double h3b = gb * (-4. * C0 * h3 * h3 * h3 * fg[0]);
+
+        // -- reverse grad-h sweep
+        double gi[6] = {0., 0., 0., 0., 0., 0.};   // x, y, z, vx, vy, vz of i
+        double hsum = 0.;
+        sweep([&](const DRec &r, int k) {
+            const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+            const double dvx = pvx - r.vx, dvy = pvy - r.vy, dvz = pvz - r.vz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double rr = __builtin_sqrt(r2);
+            const double q = rr * h3;
+            if (!(q < 1.)) return;
+            const double t = 1. - q;
+            const double t2 = t * t, t3 = t2 * t;
+            const double dw = k5 * r.m * h3_5 * t3;
+            const double P = 3. + q * (9. - 32. * q), Pp = 9. - 64. * q;
+            const double cx = dvy * dz - dvz * dy, cy = dvz * dx - dvx * dz, cz = dvx * dy - dvy * dx;
+            const double Sdw = dvsb * (dvx * dx + dvy * dy + dvz * dz) + rb[0] * cx + rb[1] * cy + rb[2] * cz;
+            // explicit d and dv terms
+            double db[3] = {dw * (dvsb * dvx + (rb[1] * dvz - rb[2] * dvy)),
+                            dw * (dvsb * dvy + (rb[2] * dvx - rb[0] * dvz)),
+                            dw * (dvsb * dvz + (rb[0] * dvy - rb[1] * dvx))};
+            const double vb[3] = {dw * (dvsb * dx + (dy * rb[2] - dz * rb[1])),
+                                  dw * (dvsb * dy + (dz * rb[0] - dx * rb[2])),
+                                  dw * (dvsb * dz + (dx * rb[1] - dy * rb[0]))};
+            // d L / d q: through gsum's m t^3 P(q) and through dw (t = 1 - q)
+            const double qb = gsb * r.m * t2 * (t * Pp - 3. * P) - 3. * Sdw * k5 * r.m * h3_5 * t2;
+            hsum += Sdw * 5. * k5 * r.m * h3_4 * t3 + qb * rr;
+            const double rb_ = qb * h3;
+            if (rr > 0.) {
+                const double f = rb_ / rr;
+                db[0] += f * dx;
+                db[1] += f * dy;
+                db[2] += f * dz;
+            }
+            gi[0] += db[0];
+            gi[1] += db[1];
+            gi[2] += db[2];
+            gi[3] += vb[0];
+            gi[4] += vb[1];
+            gi[5] += vb[2];
+            atomicAdd(&jb[0 * kTile + k], -db[0]);
+            atomicAdd(&jb[1 * kTile + k], -db[1]);
+            atomicAdd(&jb[2 * kTile + k], -db[2]);
+            atomicAdd(&jb[3 * kTile + k], -vb[0]);
+            atomicAdd(&jb[4 * kTile + k], -vb[1]);
+            atomicAdd(&jb[5 * kTile + k], -vb[2]);
+        }, true, slice, S);
+        double hs[1] = {hsum};
+        slice_total<1>(hs, red, lane, il, ni, S);
+        h3b += hs[0];
+        H3b += h3b * (-h3 * h3);
+
+        // -- reverse down the smoothing-length chain (per particle)
+        double rhob[3];
+        rhob[2] = rho2b + (capped[3] ? 0. : H3b * (-H[3] / (3. * rho2)));
+        double Hb = 0.;
+        for (int k = 2; k >= 0; k--) {
+            const double h = 1. / H[k];
+            const double drho_dh = 3. * rho[k] / h + k5 * (h * h) * (h * h) * Sk[k];
+            Hb = rhob[k] * drho_dh * (-h * h);
+            if (k > 0) rhob[k - 1] = capped[k] ? 0. : Hb * (-H[k] / (3. * rho[k - 1]));
+        }
+        // Hb is now d L / d H_0
+
+        // -- reverse of the three h-passes' pair sums (positions only)
+        double wk[3], hk[3];
+        for (int k = 0; k < 3; k++) {
+            hk[k] = 1. / H[k];
+            const double h2 = hk[k] * hk[k];
+            wk[k] = rhob[k] * k5 * (h2 * h2 * hk[k]);   // rho_bar h^5 (-5 C1)
+        }
+        sweep([&](const DRec &r, int k) {
+            const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+            const double rr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+            double c = 0.;
+            for (int q = 0; q < 3; q++) {
+                const double qq = rr * hk[q];
+                if (qq < 1.) {
+                    const double t = 1. - qq;
+                    c += wk[q] * (t * t * t);
+                }
+            }
+            if (c == 0.) return;
+            c *= r.m;
+            gi[0] += c * dx;
+            gi[1] += c * dy;
+            gi[2] += c * dz;
+            atomicAdd(&jb[0 * kTile + k], -c * dx);
+            atomicAdd(&jb[1 * kTile + k], -c * dy);
+            atomicAdd(&jb[2 * kTile + k], -c * dz);
+        }, true, slice, S);
+
+        // -- i-side totals
+        slice_total<6>(gi, red, lane, il, ni, S);
+        if (lead) {
+            atomicAdd(&grad.x[i], gi[0]);
+            atomicAdd(&grad.y[i], gi[1]);
+            atomicAdd(&grad.z[i], gi[2]);
+            atomicAdd(&grad.vx[i], gi[3]);
+            atomicAdd(&grad.vy[i], gi[4]);
+            atomicAdd(&grad.vz[i], gi[5]);
+            grad.ksr[i] += Hb;
+        }
+    }
+}
+
 }  // namespace
 }  // namespace sphk
+
+extern "C" int sphk_density_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
+                                    const int32_t *j_idx, sphk_dens_src src, sphk_dens_cot cot,
+                                    sphk_dens_grad grad, void *stream)
+{
+    using sphk::set_error;
+    if (!ctx) return set_error(SPHK_E_ARG, "density_vjp: null context");
+    if (n_walk < 0 || (n_walk > 0 && (!i_off || !j_off))) return set_error(SPHK_E_ARG, "density_vjp: bad walks");
+    if (n_walk == 0) return SPHK_OK;
+    const void *in[] = {src.id, src.m, src.x, src.y, src.z, src.vx, src.vy, src.vz, src.ksr,
+                        cot.dens, cot.ksr, cot.divv, cot.rotv, cot.grdh,
+                        grad.x, grad.y, grad.z, grad.vx, grad.vy, grad.vz, grad.ksr};
+    for (const void *q : in)
+        if (!q) return set_error(SPHK_E_ARG, "density_vjp: null pointer");
+    sphk_params p;
+    const int rc = sphk_get_params(ctx, &p);
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(sphk::density_vjp_kernel, dim3(n_walk), dim3(sphk::kWave), 0, st, n_walk, i_off, j_off,
+                       j_idx, src, p.eta * p.ksrh, p.hmax, cot, grad);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(SPHK_E_DEVICE, "density_vjp: %s", hipGetErrorString(e));
+    return SPHK_OK;
+}
 
 extern "C" int sphk_hydro_vjp_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
                                   const int32_t *j_idx, sphk_hydro_src src, sphk_hydro_out cot,

@@ -172,6 +172,17 @@ class Context:
                                        _mk(L.HydroGrad, L.HYDRO_GRAD, grad), self._stream()),
               "sphk_hydro_vjp_dev")
 
+    def density_vjp(self, w: Walks, src: dict, cot: dict, grad: dict):
+        """Add to grad (DENS_GRAD fields) the vector-Jacobian product of the
+        FP64 density functor over gather lists w (one particle set for both
+        sides), for output cotangents cot (DENS_COT fields), with this
+        context's eta, kappa and H_max."""
+        check(lib().sphk_density_vjp_dev(self._h, w.n_walk, w.i_off.data_ptr(), w.j_off.data_ptr(),
+                                         None if w.j_idx is None else w.j_idx.data_ptr(),
+                                         _mk(L.DensSrc, L.DENS_SRC, src), _mk(L.DensCot, L.DENS_COT, cot),
+                                         _mk(L.DensGrad, L.DENS_GRAD, grad), self._stream()),
+              "sphk_density_vjp_dev")
+
     def gravity(self, w: Walks, pj: dict, pi: dict, out: dict, ps: dict | None = None,
                 precision: int = FP64):
         check(lib().sphk_gravity_dev(self._h, w.n_walk, w.i_off.data_ptr(), w.j_off.data_ptr(),

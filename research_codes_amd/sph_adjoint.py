@@ -12,7 +12,8 @@ The restatement follows the functors and glue the HIP step runs:
 
 * predict / correct (hdr_hgas.hpp:646-665);
 * the density functor's three smoothing-length passes and the grad-h,
-  divergence and curl pass (hdr_density.hpp:254-363);
+  divergence and curl pass (hdr_density.hpp:254-363) on the device, through
+  its hand-written vector-Jacobian product (sphk_density_vjp_dev);
 * EOS and Balsara switch (hdr_hgas.hpp:318-344, hdr_sph.hpp:79-89);
 * tree gravity and its write-back (hdr_gravity.hpp:107-113, 295-338),
   through the hand-written vector-Jacobian product (adjoint.TreeGravity);
@@ -198,23 +199,6 @@ def _density_t(P, m, pairs, hmax, eta, ksrh):
     return dens, H, divv / dens * omg, rot / dens * omg, omg
 
 
-def _density_radius(P, m, pairs, hmax, eta, ksrh):
-    """Largest H of the three passes per particle (no gradient): list
-    entries beyond it contribute nothing to any pass."""
-    with torch.no_grad():
-        i, j = pairs
-        seg = _Seg(i, j, m.numel())
-        r = torch.sqrt(sum((P[c][i] - P[c][j]) ** 2 for c in ("x", "y", "z")))
-        H = P["ksr"].detach()
-        R = H.clone()
-        for _ in range(3):
-            hi = 1.0 / H
-            dens = seg.sum(m[j] * _w0(r * hi[i])) * (hi * hi * hi)
-            H = torch.clamp(eta * ksrh * (m / dens) ** (1.0 / 3.0), max=hmax)
-            R = torch.maximum(R, H)
-        return r, R
-
-
 def _hydro_t(P, m, pairs, d, G):
     """The hydro functor on non-self pairs (i, j) with r < max(H_i, H_j), in
     torch (the autograd reference the device VJP is tested against)."""
@@ -285,19 +269,62 @@ def hydro_dev(walks, dctx, m, ids, inputs: dict) -> dict:
     return dict(zip(_HYDRO_OUTS, out))
 
 
+_DENS_IN = ("x", "y", "z", "vx", "vy", "vz", "ksr")
+
+
+class _DensFn(torch.autograd.Function):
+    """The FP64 density functor on the device (sphk_density_dev) over gather
+    lists, at the smoothing-length cap hmax, differentiated by its
+    hand-written vector-Jacobian product (sphk_density_vjp_dev).  Inputs:
+    positions, velocities and the starting H (_DENS_IN order); outputs dens,
+    ksr (= H_3), divv, rotv, grdh."""
+
+    @staticmethod
+    def _run(dctx, hmax, f):
+        keep = dctx.params["hmax"]
+        dctx.set_params(hmax=hmax)
+        try:
+            f()
+        finally:
+            dctx.set_params(hmax=keep)
+
+    @staticmethod
+    def forward(ctx, walks, dctx, hmax, m, ids, *vals):
+        src = dict(zip(_DENS_IN, (v.contiguous() for v in vals)))
+        src.update(id=ids, m=m, rs=src["ksr"])
+        out = {k: torch.empty_like(m) for k in ("dens", "divv", "rotv", "grdh", "ksr")}
+        out["np"] = torch.empty(m.numel(), dtype=torch.int64, device=m.device)
+        out["itr"] = torch.empty(m.numel(), dtype=torch.uint8, device=m.device)
+        _DensFn._run(dctx, hmax, lambda: dctx.density(walks, src, src, out, FP64))
+        ctx.save_for_backward(*(src[k] for k in _DENS_IN))
+        ctx.walks, ctx.dctx, ctx.hmax, ctx.m, ctx.ids = walks, dctx, hmax, m, ids
+        return tuple(out[k] for k in L.DENS_COT)
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        src = dict(zip(_DENS_IN, ctx.saved_tensors))
+        src.update(id=ctx.ids, m=ctx.m, rs=src["ksr"])
+        cot = {k: torch.zeros_like(ctx.m) if g is None else g.contiguous() for k, g in zip(L.DENS_COT, gouts)}
+        grad = {k: torch.zeros_like(ctx.m) for k in L.DENS_GRAD}
+        _DensFn._run(ctx.dctx, ctx.hmax, lambda: ctx.dctx.density_vjp(ctx.walks, src, cot, grad))
+        return (None,) * 5 + tuple(grad[k] for k in _DENS_IN)
+
+
+def density_dev(walks, dctx, hmax, m, ids, inputs: dict):
+    """Differentiable density functor outputs (dens, ksr, divv, rotv, grdh)
+    from positions, velocities and the starting H (inputs["ksr"])."""
+    return _DensFn.apply(walks, dctx, float(hmax), m.detach(), ids, *(inputs[k] for k in _DENS_IN))
+
+
 def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
     """One force evaluation (density, EOS + Balsara, gravity + write-back,
     hydro + write-back + rates) on the recorded lists, differentiable in
     the positions, velocities, u, alpha, alpha_u and the starting H of P."""
     ph = sysm.phys
     ksrh = ph.ksrh
-    # density over the recorded gather lists, trimmed to the entries that
-    # can contribute (r below the largest H of the three passes)
-    i, j = expand_pairs(rec["dens"])
-    r, R = _density_radius(P, m, (i, j), rec["hmax"], sysm.eta, ksrh)
-    keep = r < R[i] * (1.0 + 1e-9)
-    dens, H, divv, rotv, omg = _density_t(P, m, (i[keep], j[keep]), rec["hmax"], sysm.eta, ksrh)
-    del i, j, r, R, keep
+    # density: the device functor and its hand-written VJP over the recorded
+    # gather lists, at the step's smoothing-length cap
+    dens, H, divv, rotv, omg = density_dev(rec["dens"], sysm.ctx, rec["hmax"], m, rec["id"], P)
     # EOS and Balsara switch
     u = P["uene"]
     pres = ph.ninv * dens * u
