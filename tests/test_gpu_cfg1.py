@@ -14,11 +14,19 @@ import torch
 
 import oracle
 from research_codes_amd import ic
-from research_codes_amd.device import FP64, MIXED
+from research_codes_amd.device import FP64, MIXED, Context
 from research_codes_amd.step import SPHSystem
 from tests._cases import scaled_err
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    """A context of its own: SPHSystem sets eta, H_max and the uniform
+    softening on the context it is given."""
+    with Context(0) as c:
+        yield c
 
 
 def _host(w):

@@ -95,6 +95,7 @@ def test_gravity(ctx, case, prec):
     mom = _dev(case["grav_mom"])
     n = len(case["p"]["m"])
     out = D.empty_out(D.L.GRAV_OUT, n)
+    ctx.set_params(uniform_softening=False)      # eps_i = H_i / kappa, as the oracle
     ctx.gravity(w, src, src, out, ps=mom, precision=prec)
     torch.cuda.synchronize()
     ref = case["grav_out"]
