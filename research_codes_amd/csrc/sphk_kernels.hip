@@ -303,15 +303,16 @@ __device__ void density_streaming(int lane, const Map &M, int i, int jb, int je,
             __syncthreads();
             dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, false);
             __syncthreads();
+            const int nt = min(kTile, je - jb - t * kTile);   // records of this tile
             int k = M.slice;
-            for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+            for (; k + 3 * M.S < nt; k += 4 * M.S) {
                 const double4 j0 = sp[k], j1 = sp[k + M.S], j2 = sp[k + 2 * M.S], j3 = sp[k + 3 * M.S];
                 dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
                 dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
                 dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
                 dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
             }
-            for (; k < kTile; k += M.S) dens_pass1_pair(sp[k], I.x, I.y, I.z, hi, hi3, a);
+            for (; k < nt; k += M.S) dens_pass1_pair(sp[k], I.x, I.y, I.z, hi, hi3, a);
         }
         double v[2] = {a.rh, (double)a.nj};
         slice_reduce<2>(v, red, M, lane);
@@ -334,12 +335,13 @@ __device__ void density_streaming(int lane, const Map &M, int i, int jb, int je,
         dens_stage<MIXED>(lane, jb + t * kTile, je, j_idx, pj, sp, sv, true);
         __syncthreads();
         Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+        const int nt = min(kTile, je - jb - t * kTile);
         int k = M.slice;
-        for (; k + M.S < kTile; k += 2 * M.S) {
+        for (; k + M.S < nt; k += 2 * M.S) {
             dens_pass2_pair<R>(sp[k], sv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
             dens_pass2_pair<R>(sp[k + M.S], sv[k + M.S], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
         }
-        for (; k < kTile; k += M.S)
+        for (; k < nt; k += M.S)
             dens_pass2_pair<R>(sp[k], sv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
         acc[0] += (double)a.g;
         acc[1] += (double)a.dv;
@@ -410,8 +412,9 @@ __global__ __launch_bounds__(kWave) void density_kernel(
                 uint64_t b = 0;
                 if (t < ntile) {
                     const double4 *tp = sp + t * kTile;
+                    const int nt = min(kTile, nj - t * kTile);
                     int k = M.slice;
-                    for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+                    for (; k + 3 * M.S < nt; k += 4 * M.S) {
                         const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
                         const double x0 = px - j0.x, y0 = py - j0.y, z0 = pz - j0.z;
                         const double x1 = px - j1.x, y1 = py - j1.y, z1 = pz - j1.z;
@@ -422,7 +425,7 @@ __global__ __launch_bounds__(kWave) void density_kernel(
                         b |= (uint64_t)(x2 * x2 + y2 * y2 + z2 * z2 < rm2) << (k + 2 * M.S);
                         b |= (uint64_t)(x3 * x3 + y3 * y3 + z3 * z3 < rm2) << (k + 3 * M.S);
                     }
-                    for (; k < kTile; k += M.S) {
+                    for (; k < nt; k += M.S) {
                         const double dx = px - tp[k].x, dy = py - tp[k].y, dz = pz - tp[k].z;
                         b |= (uint64_t)(dx * dx + dy * dy + dz * dz < rm2) << k;
                     }
@@ -527,15 +530,16 @@ __global__ __launch_bounds__(kWave) void density_kernel(
             } else {
                 for (int t = 0; t < ntile; t++) {
                     const double4 *tp = sp + t * kTile;
+                    const int nt = min(kTile, nj - t * kTile);
                     int k = MF.slice;
-                    for (; k + 3 * MF.S < kTile; k += 4 * MF.S) {
+                    for (; k + 3 * MF.S < nt; k += 4 * MF.S) {
                         const double4 j0 = tp[k], j1 = tp[k + MF.S], j2 = tp[k + 2 * MF.S], j3 = tp[k + 3 * MF.S];
                         dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
                         dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
                         dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
                         dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
                     }
-                    for (; k < kTile; k += MF.S) dens_pass1_pair(tp[k], I.x, I.y, I.z, hi, hi3, a);
+                    for (; k < nt; k += MF.S) dens_pass1_pair(tp[k], I.x, I.y, I.z, hi, hi3, a);
                 }
                 v[0] = a.rh;
                 v[1] = (double)a.nj;
@@ -617,13 +621,14 @@ __global__ __launch_bounds__(kWave) void density_kernel(
                 const double4 *tp = sp + t * kTile;
                 const R4<R> *tv = sv + t * kTile;
                 Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+                const int nt = min(kTile, nj - t * kTile);   // records of this tile
                 int k = MF.slice;
-                for (; k + MF.S < kTile; k += 2 * MF.S) {
+                for (; k + MF.S < nt; k += 2 * MF.S) {
                     dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
                     dens_pass2_pair<R>(tp[k + MF.S], tv[k + MF.S], I.x, I.y, I.z, vx, vy, vz, I.id, hi,
                                        hi4, a);
                 }
-                for (; k < kTile; k += MF.S)
+                for (; k < nt; k += MF.S)
                     dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
                 acc[0] += (double)a.g;
                 acc[1] += (double)a.dv;
@@ -803,7 +808,7 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
             __syncthreads();
             HydroAcc<R> a{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), vsmx};
             int k = M.slice;
-            for (; k + M.S < kTile; k += 2 * M.S) {
+            for (; k + M.S < nt; k += 2 * M.S) {
                 const int k1 = k + M.S;
                 const PR p0 = sp[k], p1 = sp[k1];
                 const R4<R> a0 = sa[k], a1 = sa[k1], b0 = sb[k], b1 = sb[k1], c0_ = sc[k], c1 = sc[k1];
@@ -812,7 +817,7 @@ __global__ __launch_bounds__(kWave) void hydro_kernel(
                 hydro_pair<R>(p0, a0, b0, c0_, e0, d0, I, a);
                 hydro_pair<R>(p1, a1, b1, c1, e1, d1, I, a);
             }
-            for (; k < kTile; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sid[k], I, a);
+            for (; k < nt; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sid[k], I, a);
             acc[0] += (double)a.ax;
             acc[1] += (double)a.ay;
             acc[2] += (double)a.az;
@@ -932,7 +937,7 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
         __syncthreads();
         GravAcc<R> a{R(0), R(0), R(0), R(0), R(0)};
         int k = M.slice;
-        for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+        for (; k + 3 * M.S < nt; k += 4 * M.S) {
             const GRec<MIXED> p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
             if (UNIFORM) {
                 gravity_pair_uniform<MIXED>(p0, px, py, pz, e2, a);
@@ -946,7 +951,7 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
                 gravity_pair<MIXED>(p3, px, py, pz, e2, a);
             }
         }
-        for (; k < kTile; k += M.S) {
+        for (; k < nt; k += M.S) {
             if (UNIFORM)
                 gravity_pair_uniform<MIXED>(sp[k], px, py, pz, e2, a);
             else
@@ -1000,14 +1005,14 @@ __device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb,
         __syncthreads();
         GravAcc<float> a{0.f, 0.f, 0.f, 0.f, 0.f};
         int k = M.slice;
-        for (; k + 3 * M.S < kTile; k += 4 * M.S) {
+        for (; k + 3 * M.S < nt; k += 4 * M.S) {
             const float4 p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
             gravity_pair_rel(p0, xi, yi, zi, e2, a);
             gravity_pair_rel(p1, xi, yi, zi, e2, a);
             gravity_pair_rel(p2, xi, yi, zi, e2, a);
             gravity_pair_rel(p3, xi, yi, zi, e2, a);
         }
-        for (; k < kTile; k += M.S) gravity_pair_rel(sp[k], xi, yi, zi, e2, a);
+        for (; k < nt; k += M.S) gravity_pair_rel(sp[k], xi, yi, zi, e2, a);
         acc[0] += (double)a.ax;
         acc[1] += (double)a.ay;
         acc[2] += (double)a.az;
