@@ -732,9 +732,36 @@ __device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const
     a.gz += dg * dpz;
 }
 
+// One j-record of the hydro functor as staged in LDS (R: the pair-term
+// type; positions as offsets from the walk's first particle in mixed mode).
+template <typename R>
+struct HRec {
+    typename HPos<R>::rec p;
+    R4<R> a, b, c;
+    R e;
+    int32_t id;
+};
+
+// Record k of the list, or an inert record for k >= end (beyond every
+// support, zero mass, signal speed -1e20 so it never sets vsmx).
+template <typename R>
+__device__ __forceinline__ HRec<R> hydro_load(const sphk_hydro_src &pj, const int32_t *__restrict__ j_idx, int k,
+                                              int end, double cx, double cy, double cz)
+{
+    using PR = typename HPos<R>::rec;
+    if (k < end) {
+        const int j = j_idx ? j_idx[k] : k;
+        return HRec<R>{PR{(R)(pj.x[j] - cx), (R)(pj.y[j] - cy), (R)(pj.z[j] - cz), (R)pj.m[j]},
+                       R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)pj.hinv[j]},
+                       R4<R>{(R)pj.pres[j], (R)pj.presc[j], (R)pj.bswt[j], (R)pj.dens[j]},
+                       R4<R>{(R)pj.vsnd[j], (R)pj.alph[j], (R)pj.alphu[j], (R)pj.thrm[j]}, (R)pj.eta[j], pj.id[j]};
+    }
+    return HRec<R>{PR{(R)kFar, (R)kFar, (R)kFar, R(0)}, R4<R>{R(0), R(0), R(0), R(0)},
+                   R4<R>{R(0), R(0), R(0), R(1)}, R4<R>{R(-1e20), R(0), R(0), R(0)}, R(0), -1};
+}
+
 // FP64: at most 168 VGPRs (3 waves per SIMD, a few spills): 11 % faster than
-// 2 waves without spills.  Mixed stays at 3 waves (130 VGPRs); 4 waves in
-// 128 VGPRs measured 4 % slower.
+// 2 waves without spills.  Mixed needs 115 VGPRs (4 waves) as written.
 template <bool MIXED>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 3))) void hydro_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
@@ -789,25 +816,19 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
 
         double acc[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
         R vsmx = R(0);
+        // each lane loads its record of the tile into registers before the
+        // barrier (loading the next tile's record during the sweep measured
+        // 3 % slower in mixed mode, and its registers spill in FP64)
         for (int t0 = jb; t0 < je; t0 += kTile) {
             const int nt = min(kTile, je - t0);
+            const HRec<R> nx = hydro_load<R>(pj, j_idx, t0 + lane, je, cx, cy, cz);
             __syncthreads();
-            if (lane < nt) {
-                const int j = j_idx ? j_idx[t0 + lane] : t0 + lane;
-                sp[lane] = PR{(R)(pj.x[j] - cx), (R)(pj.y[j] - cy), (R)(pj.z[j] - cz), (R)pj.m[j]};
-                sa[lane] = R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)pj.hinv[j]};
-                sb[lane] = R4<R>{(R)pj.pres[j], (R)pj.presc[j], (R)pj.bswt[j], (R)pj.dens[j]};
-                sc[lane] = R4<R>{(R)pj.vsnd[j], (R)pj.alph[j], (R)pj.alphu[j], (R)pj.thrm[j]};
-                se[lane] = (R)pj.eta[j];
-                sid[lane] = pj.id[j];
-            } else {
-                sp[lane] = PR{(R)kFar, (R)kFar, (R)kFar, R(0)};
-                sa[lane] = R4<R>{R(0), R(0), R(0), R(0)};
-                sb[lane] = R4<R>{R(0), R(0), R(0), R(1)};
-                sc[lane] = R4<R>{R(-1e20), R(0), R(0), R(0)};
-                se[lane] = R(0);
-                sid[lane] = -1;
-            }
+            sp[lane] = nx.p;
+            sa[lane] = nx.a;
+            sb[lane] = nx.b;
+            sc[lane] = nx.c;
+            se[lane] = nx.e;
+            sid[lane] = nx.id;
             __syncthreads();
             HydroAcc<R> a{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), vsmx};
             int k = M.slice;
