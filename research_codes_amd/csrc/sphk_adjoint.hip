@@ -49,7 +49,7 @@ __device__ void vjp_list(int lane, int S, int slice, int kb, int ke, const int32
         jb[kTile + lane] = 0.;
         jb[2 * kTile + lane] = 0.;
         __syncthreads();
-        for (int k = slice; k < kTile; k += S) {
+        for (int k = slice; k < nt; k += S) {
             const VRec r = sp[k];
             const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
             const double s = dx * dx + dy * dy + dz * dz + e2;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kWave) void hydro_vjp_kernel(int n_walk, const int3
                                  HCot{0., 0., 0., 0., 0., 0., 0., 0.}};
             }
             __syncthreads();
-            for (int k = slice; k < kTile; k += S) {
+            for (int k = slice; k < nt; k += S) {
                 const HRecV &Q = sp[k];
                 hydro_pair_vjp<true>(P, Q.p, cp, g);
                 hydro_pair_vjp<false>(Q.p, P, Q.c, g);
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kWave) void density_vjp_kernel(int n_walk, const in
             if (flush)
                 for (int q = 0; q < 6; q++) jb[q * kTile + lane] = 0.;
             __syncthreads();
-            for (int k = slice; k < kTile; k += S) f(sp[k], k);
+            for (int k = slice; k < nt; k += S) f(sp[k], k);
             if (flush) {
                 __syncthreads();
                 if (lane < nt) {
