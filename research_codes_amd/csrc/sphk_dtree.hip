@@ -592,6 +592,44 @@ __device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&
     return m.child < 0 ? kLeaf : kOpen;
 }
 
+// One lane's output stream of int32 entries, written in aligned 16-byte
+// pieces: entries before the first multiple-of-4 position go out one by one,
+// then four at a time from registers, the rest at flush().  The long list's
+// entries arrive one per tree visit; single 4-byte stores from 64 walks at
+// once leave partly written cache lines that the memory side completes one
+// sector per entry.
+struct Out4 {
+    int32_t *out;
+    int64_t pos;   // next position; entries [pos - n, pos) wait in b0..b3
+    int n;
+    int32_t b0, b1, b2, b3;
+    __device__ __forceinline__ void put(int32_t v)
+    {
+        if (n == 0 && (pos & 3) != 0) {
+            out[pos++] = v;
+            return;
+        }
+        b0 = n == 0 ? v : b0;
+        b1 = n == 1 ? v : b1;
+        b2 = n == 2 ? v : b2;
+        b3 = n == 3 ? v : b3;
+        n++;
+        pos++;
+        if (n == 4) {
+            *reinterpret_cast<int4 *>(out + pos - 4) = make_int4(b0, b1, b2, b3);
+            n = 0;
+        }
+    }
+    __device__ __forceinline__ void flush()
+    {
+        const int64_t p = pos - n;
+        if (n > 0) out[p] = b0;
+        if (n > 1) out[p + 1] = b1;
+        if (n > 2) out[p + 2] = b2;
+        n = 0;
+    }
+};
+
 // The host builder's depth-first walk of one group, split at the frontier:
 // frontier node f (the nodes of one level plus the leaves above it, in
 // particle order) gets its own thread per walk.  The thread first replays
@@ -632,6 +670,8 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
     int64_t nj = 0, ns = 0;
     const int64_t j0 = FILL ? oj[tid] : 0;
     const int64_t s0 = (FILL && MODE == 2) ? os[tid] : 0;
+    // long list: both streams through write-combining buffers
+    Out4 jw{jout, j0, 0, 0, 0, 0, 0}, sw{sout, s0, 0, 0, 0, 0, 0};
 
     // ancestors of f, root first (the frontier is a few levels deep)
     const int lf = t.nd.level[f];
@@ -644,7 +684,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
         if (v == kOpen) continue;
         reach = false;
         if (v == kSpj && ln.begin == t.nd.begin[f]) {
-            if (FILL) sout[s0 + ns] = a;
+            if (FILL) sw.put(a);
             ns++;
         }
     }
@@ -662,7 +702,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
             nxt = ln.next;
             if (v == kSkip) continue;
             if (v == kSpj) {
-                if (FILL) sout[s0 + ns] = a;
+                if (FILL) sw.put(a);
                 ns++;
                 continue;
             }
@@ -674,7 +714,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
             // leaf: its particles, filtered by exact membership for the short lists
             for (int j = ln.begin; j < ln.end; j++) {
                 if (MODE == 2) {
-                    if (FILL) jout[j0 + nj] = j;
+                    if (FILL) jw.put(j);
                     nj++;
                     continue;
                 }
@@ -697,6 +737,10 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
                 }
             }
         }
+    }
+    if (FILL && MODE == 2) {
+        jw.flush();
+        sw.flush();
     }
     if (!FILL) {
         cj[tid] = nj;
