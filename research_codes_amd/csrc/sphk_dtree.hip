@@ -645,7 +645,7 @@ __global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, co
                                                   double inv_t2, int64_t *__restrict__ cj,
                                                   int64_t *__restrict__ cs, const int64_t *__restrict__ oj,
                                                   const int64_t *__restrict__ os, int32_t *__restrict__ jout,
-                                                  int32_t *__restrict__ sout, int *__restrict__ err)
+                                                  int32_t *__restrict__ sout)
 {
 #pragma clang fp contract(off)
     // neighbouring lanes take neighbouring walks under the same frontier
@@ -1016,7 +1016,6 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
     SPHK_TRY(t->i64c.reserve(nt, false, t->st));   // EPJ offsets
     SPHK_TRY(t->i64d.reserve(nt, false, t->st));   // SPJ offsets
     SPHK_TRY(t->total.reserve(2, false, t->st));
-    SPHK_TRY(t->err.reserve(1, false, t->st));
     SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
     if (MODE == 2) SPHK_TRY(L.s_off.reserve(ng + 1, false, t->st));
     if (MODE != 2) {
@@ -1024,12 +1023,11 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
         hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p,
                            t->y.p, t->z.p, r, t->wbox.p);
     }
-    SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
     TreeView v{t->nodes(), t->moments(), t->brec.p, t->mrec.p, t->orec.p, t->x.p, t->y.p, t->z.p, t->group.p,
                t->i_off.p};
     const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
     hipLaunchKernelGGL((list_kernel<MODE, false>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
-                       t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
+                       t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr);
     SPHK_TRY(hipGetLastError());
     int64_t tot[2] = {0, 0};
     if (nt > std::numeric_limits<int>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many walk subtrees");
@@ -1049,11 +1047,8 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
     if (MODE == 2) SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
     hipLaunchKernelGGL((list_kernel<MODE, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
                        nullptr, nullptr, t->i64c.p, MODE == 2 ? t->i64d.p : nullptr, L.j_idx.p,
-                       MODE == 2 ? L.s_idx.p : nullptr, t->err.p);
+                       MODE == 2 ? L.s_idx.p : nullptr);
     SPHK_TRY(hipGetLastError());
-    int e = 0;
-    if (int rc = read1(t->st, t->err.p, &e)) return rc;
-    if (e) return set_error(SPHK_E_STATE, "dtree: traversal stack exceeded");
     L.n_j = tot[0];
     L.n_s = MODE == 2 ? tot[1] : 0;
     L.built = true;
@@ -1074,18 +1069,16 @@ int build_list_two_stage(sphk_dtree *t, const double *r)
     SPHK_TRY(t->i64a.reserve(nt, false, t->st));   // leaf counts per (walk, frontier node)
     SPHK_TRY(t->i64c.reserve(nt, false, t->st));   // their scan
     SPHK_TRY(t->total.reserve(2, false, t->st));
-    SPHK_TRY(t->err.reserve(1, false, t->st));
     SPHK_TRY(t->leaf_off.reserve(ng + 1, false, t->st));
     SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
     SPHK_TRY(t->wbox.reserve(6 * (size_t)ng, false, t->st));
     hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p, t->y.p,
                        t->z.p, r, t->wbox.p);
-    SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
     TreeView v{t->nodes(), t->moments(), t->brec.p, t->mrec.p, t->orec.p, t->x.p, t->y.p, t->z.p, t->group.p,
                t->i_off.p};
     const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
     hipLaunchKernelGGL((list_kernel<MODE, false, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
-                       t->i64a.p, nullptr, nullptr, nullptr, nullptr, nullptr, t->err.p);
+                       t->i64a.p, nullptr, nullptr, nullptr, nullptr, nullptr);
     SPHK_TRY(hipGetLastError());
     if (int rc = scan64(t, t->i64a.p, t->i64c.p, (int)nt)) return rc;
     hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64c.p, t->i64a.p,
@@ -1095,7 +1088,7 @@ int build_list_two_stage(sphk_dtree *t, const double *r)
     if (n_leaves > std::numeric_limits<int32_t>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many leaves");
     SPHK_TRY(t->leaf_idx.reserve(std::max<int64_t>(n_leaves, 1), false, t->st));
     hipLaunchKernelGGL((list_kernel<MODE, true, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
-                       nullptr, nullptr, t->i64c.p, nullptr, t->leaf_idx.p, nullptr, t->err.p);
+                       nullptr, nullptr, t->i64c.p, nullptr, t->leaf_idx.p, nullptr);
     SPHK_TRY(hipGetLastError());
     // stage 2: exact membership, one wave per walk, in one pass: the accepted
     // particles go to the walk's stretch of a scratch array (room for all its
@@ -1127,9 +1120,6 @@ int build_list_two_stage(sphk_dtree *t, const double *r)
     hipLaunchKernelGGL(member_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->i64d.p, L.j_off.p, t->mscr.p,
                        L.j_idx.p);
     SPHK_TRY(hipGetLastError());
-    int e = 0;
-    if (int rc = read1(t->st, t->err.p, &e)) return rc;
-    if (e) return set_error(SPHK_E_STATE, "dtree: traversal stack exceeded");
     L.n_j = tot;
     L.n_s = 0;
     L.built = true;
