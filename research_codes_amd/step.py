@@ -71,7 +71,10 @@ class SPHSystem:
                                for a, b in zip(self.seg[:-1], self.seg[1:])])
         self.n = n = len(perm)
         f64 = dict(dtype=torch.float64, device=device)
-        self.s = {k: torch.zeros(n, **f64) for k in L.STATE_FIELDS}
+        # the state fields are the rows of one (fields, n) array: a reorder
+        # permutes them all in one gather
+        self._sbuf = torch.zeros(len(L.STATE_FIELDS), n, **f64)
+        self.s = {k: self._sbuf[q] for q, k in enumerate(L.STATE_FIELDS)}
         for k in ("m", "x", "y", "z", "vx", "vy", "vz", "uene", "ksr", "alph", "alphu"):
             self.s[k].copy_(torch.from_numpy(np.ascontiguousarray(p[k][perm], np.float64)))
         self.id = to_device(np.ascontiguousarray(p["id"][perm], np.int32), device)
@@ -148,8 +151,7 @@ class SPHSystem:
                 return
             pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
         self.last_perm = pt.long()
-        for k in L.STATE_FIELDS:
-            self.s[k].copy_(self.s[k][pt])
+        self._sbuf.copy_(self._sbuf[:, self.last_perm])
         for t in (self.id, self.istar, self.rs, self.np_, self.itr, self.eps2):
             t.copy_(t[pt])
 
