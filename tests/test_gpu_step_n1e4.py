@@ -34,7 +34,7 @@ def _host(w):
 
 
 @pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
-def test_cfg1_force_step_matches_oracle(ctx, prec):
+def test_n1e4_force_step_matches_oracle(ctx, prec):
     p = ic.polytrope(1.5, 10_000, 1.0, 1.0, seed=1001, eta=1.18)
     rng = np.random.default_rng(7)
     p["vx"] = -0.2 * p["y"] + 0.02 * rng.normal(size=len(p["m"]))
@@ -47,7 +47,7 @@ def test_cfg1_force_step_matches_oracle(ctx, prec):
     s.build_lists(("dens",))
     st = {k: v.cpu().numpy().copy() for k, v in s.s.items()}
     ids, rs = s.id.cpu().numpy(), s.rs.cpu().numpy()
-    print(f"[cfg1] N={s.n} precision={prec}", flush=True)
+    print(f"[n1e4] N={s.n} precision={prec}", flush=True)
 
     # the device step
     s.density()
@@ -90,7 +90,7 @@ def test_cfg1_force_step_matches_oracle(ctx, prec):
     worst = {}
     for k in ("ax", "ay", "az", "udot", "adot", "pot", "divv", "rotv"):
         worst[k] = scaled_err(got[k], st[k], floor_frac=floor)
-    print(f"[cfg1] {prec}: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()))
+    print(f"[n1e4] {prec}: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()))
     assert abs(st["udot"]).max() > 0 and abs(st["divv"]).max() > 0
     for k, v in worst.items():
         assert v < tol, k
