@@ -31,6 +31,7 @@
  *    pair term in FP32 and accumulates per-particle totals in FP64.  The
  *    smoothing-length iteration (neighbour counts, H, itr, density) is
  *    always evaluated in FP64, so np/ksr/itr are identical in both modes.
+ *    Any other value is rejected with SPHK_E_ARG.
  */
 #ifndef SPHK_H
 #define SPHK_H

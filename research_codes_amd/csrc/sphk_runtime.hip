@@ -190,6 +190,13 @@ int check_ctx(sphk_ctx *ctx)
     return SPHK_OK;
 }
 
+int check_precision(int precision)
+{
+    if (precision != SPHK_FP64 && precision != SPHK_MIXED)
+        return set_error(SPHK_E_ARG, "precision must be SPHK_FP64 or SPHK_MIXED (got %d)", precision);
+    return SPHK_OK;
+}
+
 int check_tag(sphk_ctx *ctx, int tag)
 {
     if (tag < 0 || tag >= SPHK_TAG_MAX)
@@ -401,6 +408,7 @@ int sphk_density_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    if ((rc = check_precision(precision))) return rc;
     if (n_walk < 0 || (n_walk > 0 && (!i_off || !j_off)))
         return set_error(SPHK_E_ARG, "density: bad walk table");
     if (n_walk == 0) return SPHK_OK;
@@ -423,6 +431,7 @@ int sphk_hydro_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    if ((rc = check_precision(precision))) return rc;
     if (n_walk < 0 || (n_walk > 0 && (!i_off || !j_off)))
         return set_error(SPHK_E_ARG, "hydro: bad walk table");
     if (n_walk == 0) return SPHK_OK;
@@ -449,6 +458,7 @@ int sphk_gravity_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    if ((rc = check_precision(precision))) return rc;
     if (n_walk < 0 || (n_walk > 0 && (!i_off || !j_off)))
         return set_error(SPHK_E_ARG, "gravity: bad walk table");
     if (n_walk == 0) return SPHK_OK;
@@ -473,6 +483,7 @@ int sphk_density_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    if ((rc = check_precision(precision))) return rc;
     if ((rc = check_tag(ctx, tag))) return rc;
     if (n_walk < 0 || n_walk > SPHK_WALK_LIMIT)
         return set_error(SPHK_E_CAPACITY, "n_walk %d outside [0,%d]", n_walk, SPHK_WALK_LIMIT);
@@ -511,6 +522,7 @@ int sphk_hydro_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip, 
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    if ((rc = check_precision(precision))) return rc;
     if ((rc = check_tag(ctx, tag))) return rc;
     if (n_walk < 0 || n_walk > SPHK_WALK_LIMIT)
         return set_error(SPHK_E_CAPACITY, "n_walk %d outside [0,%d]", n_walk, SPHK_WALK_LIMIT);
@@ -550,6 +562,7 @@ int sphk_gravity_dispatch(sphk_ctx *ctx, int tag, int n_walk, const int32_t *nip
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    if ((rc = check_precision(precision))) return rc;
     if ((rc = check_tag(ctx, tag))) return rc;
     if (n_walk < 0 || n_walk > SPHK_WALK_LIMIT)
         return set_error(SPHK_E_CAPACITY, "n_walk %d outside [0,%d]", n_walk, SPHK_WALK_LIMIT);

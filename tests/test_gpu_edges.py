@@ -131,3 +131,14 @@ def test_tiny_gravity_walks_match_oracle(ctx, prec):
         e = scaled_err(got[k], ref[k], floor_frac=1e-1 if k in ("ax", "ay", "az") else 1e-3)
         print(f"[edges] gravity[{prec}] {k}: {e:.2e}")
         assert e < tol, k
+
+
+def test_unknown_precision_is_rejected(ctx):
+    p, _ = _particles()
+    src = _dev({k: p[k] for k in D.L.DENS_SRC})
+    out = D.empty_dens_out(len(p["m"]))
+    with pytest.raises(D.L.SphkError, match="precision"):
+        ctx.density(D.Walks(I_OFF, J_OFF, J_IDX), src, src, out, 2)
+    g = _dev({"m": p["m"], "x": p["x"], "y": p["y"], "z": p["z"], "eps2": (p["ksr"] / KSRH) ** 2})
+    with pytest.raises(D.L.SphkError, match="precision"):
+        ctx.gravity(D.Walks(I_OFF, J_OFF, J_IDX), g, g, D.empty_out(D.L.GRAV_OUT, len(p["m"])), precision=-1)
