@@ -28,11 +28,41 @@ __device__ __forceinline__ int xcd_walk_adj(int b, int n_walk)
     return ((k / kC) * kX + x) * kC + k % kC;
 }
 
-__device__ void vjp_list(int lane, int S, int slice, int kb, int ke, const int32_t *__restrict__ idx,
-                         const sphk_grav_src &src, double px, double py, double pz, double e2, double ax,
-                         double ay, double az, double pb, double eb, VRec *sp, double *jb, int32_t *sj,
-                         double (&gi)[3],
-                         double *outx, double *outy, double *outz)
+// One i-particle's position and output cotangents, as the j-side sweep reads
+// them from LDS.
+struct VI {
+    double x, y, z, ax, ay, az, pb, eb;
+};
+
+// Cotangent of record r's position from the pair (i, r): the reverse of the
+// uniform-softening pair term, m_r (T abar - pbar w^3 d + 3 ebar w^5 d) with
+// T = w^3 I - 3 w^5 d d^T and d = x_i - x_r; the i-side gets its negative.
+__device__ __forceinline__ void vjp_pair(const VI &I, const VRec &r, double e2, double &cx, double &cy,
+                                         double &cz)
+{
+    const double dx = I.x - r.x, dy = I.y - r.y, dz = I.z - r.z;
+    const double s = dx * dx + dy * dy + dz * dz + e2;
+    const double w = rsqrt_f64(s);
+    const double w3 = w * w * w;
+    const double w5 = w3 * w * w;
+    const double dot = dx * I.ax + dy * I.ay + dz * I.az;
+    // T abar = w3 abar - 3 w5 d (d . abar); then subtract pbar w3 d
+    // and add 3 ebar w5 d (the eta sum's gradient is -3 m w5 d)
+    const double f = 3. * w5 * (dot - I.eb) + I.pb * w3;
+    cx = r.m * (w3 * I.ax - f * dx);
+    cy = r.m * (w3 * I.ay - f * dy);
+    cz = r.m * (w3 * I.az - f * dz);
+}
+
+// One list of a walk chunk, tile by tile, in two sweeps over the staged
+// records: the i-side (lane = (i, slice), register accumulators) and the
+// j-side (lane = record, looping over the chunk's i-particles in LDS, one
+// global atomic per record and component).  Every pair is evaluated twice;
+// summing the j-side in LDS instead needs atomics that the n_i lanes of a
+// slice issue to the same record at once.
+__device__ void vjp_list(int lane, int S, int slice, int ni, int kb, int ke, const int32_t *__restrict__ idx,
+                         const sphk_grav_src &src, const VI &I, const VI *sI, double e2, VRec *sp, int32_t *sj,
+                         double (&gi)[3], double *outx, double *outy, double *outz)
 {
     for (int t0 = kb; t0 < ke; t0 += kTile) {
         const int nt = min(kTile, ke - t0);
@@ -41,41 +71,29 @@ __device__ void vjp_list(int lane, int S, int slice, int kb, int ke, const int32
             const int j = idx ? idx[t0 + lane] : t0 + lane;
             sp[lane] = VRec{src.x[j], src.y[j], src.z[j], src.m[j]};
             sj[lane] = j;
-        } else {
-            sp[lane] = VRec{1e10, 1e10, 1e10, 0.};
-            sj[lane] = -1;
         }
-        jb[lane] = 0.;
-        jb[kTile + lane] = 0.;
-        jb[2 * kTile + lane] = 0.;
         __syncthreads();
         for (int k = slice; k < nt; k += S) {
-            const VRec r = sp[k];
-            const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
-            const double s = dx * dx + dy * dy + dz * dz + e2;
-            const double w = rsqrt_f64(s);
-            const double w3 = w * w * w;
-            const double w5 = w3 * w * w;
-            const double dot = dx * ax + dy * ay + dz * az;
-            // T abar = w3 abar - 3 w5 d (d . abar); then subtract pbar w3 d
-            // and add 3 ebar w5 d (the eta sum's gradient is -3 m w5 d)
-            const double f = 3. * w5 * (dot - eb) + pb * w3;
-            const double cx = r.m * (w3 * ax - f * dx);
-            const double cy = r.m * (w3 * ay - f * dy);
-            const double cz = r.m * (w3 * az - f * dz);
+            double cx, cy, cz;
+            vjp_pair(I, sp[k], e2, cx, cy, cz);
             gi[0] -= cx;
             gi[1] -= cy;
             gi[2] -= cz;
-            atomicAdd(&jb[k], cx);
-            atomicAdd(&jb[kTile + k], cy);
-            atomicAdd(&jb[2 * kTile + k], cz);
         }
-        __syncthreads();
         if (lane < nt) {
+            const VRec r = sp[lane];
+            double jx = 0., jy = 0., jz = 0.;
+            for (int q = 0; q < ni; q++) {
+                double cx, cy, cz;
+                vjp_pair(sI[q], r, e2, cx, cy, cz);
+                jx += cx;
+                jy += cy;
+                jz += cz;
+            }
             const int j = sj[lane];
-            atomicAdd(&outx[j], jb[lane]);
-            atomicAdd(&outy[j], jb[kTile + lane]);
-            atomicAdd(&outz[j], jb[2 * kTile + lane]);
+            atomicAdd(&outx[j], jx);
+            atomicAdd(&outy[j], jy);
+            atomicAdd(&outz[j], jz);
         }
     }
 }
@@ -89,7 +107,7 @@ __global__ __launch_bounds__(kWave) void gravity_vjp_kernel(
     double *nx, double *ny, double *nz)
 {
     __shared__ VRec sp[kTile];
-    __shared__ double jb[3 * kTile];
+    __shared__ VI sI[kWave];
     __shared__ int32_t sj[kTile];
     __shared__ double red[3 * kWave];
     if ((int)blockIdx.x >= n_walk) return;
@@ -101,15 +119,13 @@ __global__ __launch_bounds__(kWave) void gravity_vjp_kernel(
         const int S = kWave / ni, il = lane % ni;
         const int slice = lane / ni < S ? lane / ni : kTile;   // idle lanes sweep nothing
         const int i = c0 + il;
-        const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
-        const double ax = abx[i], ay = aby[i], az = abz[i], pb = pbar[i];
-        const double eb = ebar ? ebar[i] : 0.;
+        const VI I{pi.x[i], pi.y[i], pi.z[i], abx[i], aby[i], abz[i], pbar[i], ebar ? ebar[i] : 0.};
+        __syncthreads();   // sI of the previous chunk no longer read
+        if (lane < ni) sI[lane] = I;
         double gi[3] = {0., 0., 0.};
-        vjp_list(lane, S, slice, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, ax, ay, az, pb, eb, sp, jb,
-                 sj, gi, gx, gy, gz);
+        vjp_list(lane, S, slice, ni, j_off[w], j_off[w + 1], j_idx, pj, I, sI, e2, sp, sj, gi, gx, gy, gz);
         if (s_off)
-            vjp_list(lane, S, slice, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2, ax, ay, az, pb, eb, sp,
-                     jb, sj, gi, nx, ny, nz);
+            vjp_list(lane, S, slice, ni, s_off[w], s_off[w + 1], s_idx, ps, I, sI, e2, sp, sj, gi, nx, ny, nz);
         __syncthreads();
         for (int q = 0; q < 3; q++) red[q * kWave + lane] = gi[q];
         __syncthreads();
