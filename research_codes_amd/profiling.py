@@ -11,13 +11,16 @@ reduces them to the figures the benchmark reports:
   device-to-device copy of known size (bench.py --copy-calibration runs three
   256 MiB copies first);
 * any other counters of one `--pmc` pass per launch of one kernel (e.g. the
-  hardware-counted VALU flops beside the algorithmic ones).
+  hardware-counted VALU flops beside the algorithmic ones);
+* the kernel time of the last K full KDK steps of a run (each step starts
+  at a predict launch), by stage: list builds, tree rebuild, functors, glue.
 
 Usage:
   python -m research_codes_amd.profiling trace <kernel_stats.csv> <kernel_trace.csv> [last]
   python -m research_codes_amd.profiling traffic <fetch counter_collection.csv> \
       <write counter_collection.csv> <kernel substring> [last]
   python -m research_codes_amd.profiling counters <counter_collection.csv> <kernel substring> [last]
+  python -m research_codes_amd.profiling kdk <kernel_trace.csv> [last]
 """
 from __future__ import annotations
 
@@ -106,6 +109,47 @@ def counters(csv_path: str, kernel: str, last: int = 5) -> dict:
     return {k: statistics.fmean(r.get(k, 0.0) for r in rows) for k in names}
 
 
+_KDK_STAGES = (
+    ("long list (EPJ + SPJ): count + fill", ("list_kernel<2",)),
+    ("symmetry list: candidate leaves + exact members", ("list_kernel<1", "member_kernel<1")),
+    ("gather list: candidate leaves + exact members", ("list_kernel<0", "member_kernel<0")),
+    ("list bookkeeping (candidate totals, copies, walk boxes, offsets)",
+     ("cand_total", "member_copy", "walk_box", "offsets_kernel")),
+    ("scans and sorts (rocPRIM)", ("rocprim",)),
+    ("tree rebuild (keys, levels, moments, node records, frontier)",
+     ("split_kernel", "moments_kernel", "obox_kernel", "next_kernel", "pack_", "bbox_", "keys_kernel",
+      "unsorted_kernel", "root_kernel", "group_", "front_", "head_flag", "fill_kernel")),
+    ("density functor", ("density_kernel",)),
+    ("gravity functor", ("gravity_kernel",)),
+    ("hydro functor", ("hydro_kernel",)),
+    ("step glue (EOS, write-backs, predict/correct, softening, reductions)", ("sphk::",)),
+    ("copies and fills", ("copyBuffer", "fillBuffer")),
+)
+
+
+def kdk_stages(trace_csv: str, last: int = 5) -> tuple[float, float, list[tuple[str, float, float]]]:
+    """Wall time and kernel time per KDK step over the last `last` steps
+    (from the last `last` predict launches to the end of the trace), and per
+    stage (calls per step, ms per step); kernels matching no stage are torch's
+    own (reorder gathers, comparisons, list copies)."""
+    with open(trace_csv, newline="") as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if "predict_kernel" in r["Kernel_Name"]]
+    if len(starts) < last:
+        raise ValueError(f"{len(starts)} KDK steps in the trace, {last} asked for")
+    seg = rows[starts[-last]:]
+    wall = (int(seg[-1]["End_Timestamp"]) - int(seg[0]["Start_Timestamp"])) / last / 1e6
+    acc, cnt = defaultdict(float), defaultdict(int)
+    for r in seg:
+        name = r["Kernel_Name"]
+        stage = next((s for s, keys in _KDK_STAGES if any(k in name for k in keys)),
+                     "torch ops (reorder gathers, comparisons, list copies)")
+        acc[stage] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+        cnt[stage] += 1
+    out = sorted(((s, cnt[s] / last, acc[s] / last) for s in acc), key=lambda t: -t[2])
+    return wall, sum(acc.values()) / last, out
+
+
 def main(argv: list[str]) -> int:
     if len(argv) >= 3 and argv[0] == "trace":
         last = int(argv[3]) if len(argv) > 3 else 5
@@ -130,6 +174,15 @@ def main(argv: list[str]) -> int:
         print(f"kernel={argv[2]}")
         for k, v in counters(argv[1], argv[2], last).items():
             print(f"{k}={v:.6g}")
+        return 0
+    if len(argv) >= 2 and argv[0] == "kdk":
+        last = int(argv[2]) if len(argv) > 2 else 5
+        wall, kern, stages = kdk_stages(argv[1], last)
+        print(f"per KDK step (last {last}): wall {wall:.2f} ms, kernels {kern:.2f} ms")
+        print("| stage | launches per step | ms per step |")
+        print("|---|---|---|")
+        for s_, c, ms in stages:
+            print(f"| {s_} | {c:.0f} | {ms:.2f} |")
         return 0
     print(__doc__, file=sys.stderr)
     return 2
