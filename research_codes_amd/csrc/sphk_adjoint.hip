@@ -1,12 +1,14 @@
-// sphk_adjoint.hip -- vector-Jacobian product of the gravity functor
-// (uniform softening), the reverse of gravity_pair_uniform in FP64.
+// sphk_adjoint.hip -- vector-Jacobian products of the three functors in
+// FP64: gravity (uniform softening, the reverse of gravity_pair_uniform),
+// hydro and density.
 //
-// Layout as the forward kernel: one wave per walk, S = floor(64/ni) j-slices,
-// 64-record LDS tiles.  The i-side cotangent of a lane stays in registers
-// and is combined across slices through LDS at the end of the walk; the
-// j-side contribution of a pair goes to a per-tile LDS accumulator of that
-// record (LDS atomics), flushed to HBM with one atomic add per record and
-// component after the tile.
+// Layout as the forward kernels: one wave per walk, S = floor(64/ni)
+// j-slices, 64-record LDS tiles.  The i-side cotangent of a lane stays in
+// registers and is combined across slices through LDS at the end of the
+// walk.  The j-side: gravity sweeps each tile a second time with one record
+// per lane; density collects it per tile in LDS (atomic adds); hydro needs
+// none (its symmetric lists hold both pair orders).  Either way it leaves
+// with one HBM atomic add per record and component.
 #include "sphk_adjoint.h"
 #include "sphk_device.hpp"
 #include "sphk_internal.hpp"
