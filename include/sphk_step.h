@@ -55,6 +55,11 @@ typedef struct {
     double eps_u;       /* RP::EpsilonOfInternalEnergy */
     double c_dt;        /* RP::CoefficientOfTimestep */
     double dt_max;      /* RP::MaximumTimestep */
+    double eps_const;   /* > 0: the gravitational softening is this constant
+                           length (not the reference's eps = H/kappa,
+                           hdr_gravity.hpp:136), so the potential does not
+                           depend on H and the gravity write-back sets the
+                           grad-h softening term eta to 0; <= 0: eps = H/kappa */
 } sphk_phys;
 
 /* EOS and Balsara switch.  The density functor's outputs are expected in
@@ -66,7 +71,8 @@ int sphk_eos_balsara_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, void *
 int sphk_softening_dev(sphk_ctx *ctx, int n, const double *ksr, double ksrh, double eps_const,
                        double *eps2, void *stream);
 
-/* accg2, pot and eta from the gravity functor's outputs. */
+/* accg2, pot and eta from the gravity functor's outputs (eta = 0 when
+ * p.eps_const > 0). */
 int sphk_gravity_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p,
                                sphk_grav_out g, void *stream);
 

@@ -114,7 +114,8 @@ class State(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("ninv", "ksrh", "G", "alpha_max", "alpha_min",
-                                          "alphu_max", "alphu_min", "eps_u", "c_dt", "dt_max")]
+                                          "alphu_max", "alphu_min", "eps_u", "c_dt", "dt_max",
+                                          "eps_const")]
 
 
 def kernel0th(q: float) -> float:
@@ -209,7 +210,7 @@ def _state(s: dict):
 
 def params(**kw) -> Params:
     d = dict(ninv=1.0 / 1.5, ksrh=1.936492, G=1.0, alpha_max=2.0, alpha_min=0.1, alphu_max=0.0,
-             alphu_min=0.0, eps_u=0.0, c_dt=0.1, dt_max=10.0)
+             alphu_min=0.0, eps_u=0.0, c_dt=0.1, dt_max=10.0, eps_const=0.0)
     d.update(kw)
     return Params(**d)
 

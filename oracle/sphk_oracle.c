@@ -310,7 +310,10 @@ void orc_gravity_writeback(int n, orc_state s, orc_params p,
         s.g2x[i] = p.G * gax[i];
         s.g2y[i] = p.G * gay[i];
         s.g2z[i] = p.G * gaz[i];
-        s.eta[i] = s.ksr[i] * s.ksr[i] * ksrhinv * ksrhinv * s.grdh[i] / (3. * s.dens[i]) * geta[i];
+        /* eta = H^2/kappa^2 Omega/(3 rho) sum m (r^2+eps^2)^-3/2 is dPhi/dH
+         * through eps = H/kappa; a constant eps leaves Phi independent of H */
+        s.eta[i] = (p.eps_const > 0.) ? 0.
+                 : s.ksr[i] * s.ksr[i] * ksrhinv * ksrhinv * s.grdh[i] / (3. * s.dens[i]) * geta[i];
         s.pot[i] = p.G * gpot[i];
     }
 }

@@ -115,6 +115,10 @@ typedef struct {
     double eps_u;       /* RP::EpsilonOfInternalEnergy */
     double c_dt;        /* RP::CoefficientOfTimestep */
     double dt_max;      /* RP::MaximumTimestep */
+    double eps_const;   /* > 0: constant gravitational softening length (this
+                           build's mode; the reference only has eps = H/kappa,
+                           hdr_gravity.hpp:136).  The potential then does not
+                           depend on H, so the grad-h softening term eta is 0. */
 } orc_params;
 
 /* EOS + Balsara: referEquationOfState (hdr_hgas.hpp:318-344) then
@@ -124,7 +128,8 @@ void orc_eos_balsara(int n, orc_state s, orc_params p);
 void orc_hydro_prep(int n, const double *ksr, const double *pres, const double *grdh,
                     const double *dens, const double *bswt, const double *uene,
                     double *hinv, double *presc, double *bswt_half, double *thrm);
-/* Gravity write-back SPH::copyFromForce(Gravity) (hdr_gravity.hpp:107-113). */
+/* Gravity write-back SPH::copyFromForce(Gravity) (hdr_gravity.hpp:107-113);
+ * eta = 0 when p.eps_const > 0. */
 void orc_gravity_writeback(int n, orc_state s, orc_params p,
                            const double *gax, const double *gay, const double *gaz,
                            const double *gpot, const double *geta);

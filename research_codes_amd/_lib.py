@@ -68,7 +68,8 @@ State = _struct("State", STATE_FIELDS)
 
 class Phys(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("ninv", "ksrh", "G", "alpha_max", "alpha_min",
-                                          "alphu_max", "alphu_min", "eps_u", "c_dt", "dt_max")]
+                                          "alphu_max", "alphu_min", "eps_u", "c_dt", "dt_max",
+                                          "eps_const")]
 
 
 # name -> (restype, argtypes); every symbol declared in include/*.h

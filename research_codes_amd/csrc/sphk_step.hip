@@ -54,7 +54,9 @@ __global__ __launch_bounds__(kBlock) void gravity_writeback_kernel(int n, sphk_s
     s.g2x[i] = p.G * g.ax[i];
     s.g2y[i] = p.G * g.ay[i];
     s.g2z[i] = p.G * g.az[i];
-    s.eta[i] = h * h * kinv * kinv * s.grdh[i] / (3. * s.dens[i]) * g.eta[i];  // hdr_gravity.hpp:110-111
+    // hdr_gravity.hpp:110-111: eta is dPhi/dH through eps = H/kappa; with a
+    // constant softening the potential does not depend on H and eta is 0
+    s.eta[i] = p.eps_const > 0. ? 0. : h * h * kinv * kinv * s.grdh[i] / (3. * s.dens[i]) * g.eta[i];
     s.pot[i] = p.G * g.pot[i];
 }
 

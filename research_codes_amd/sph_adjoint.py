@@ -334,8 +334,10 @@ def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
     # gravity (uniform softening) and its write-back
     tg = TreeGravity(sysm.ctx, m.detach(), eps=sysm.eps, theta=sysm.theta, n_group=sysm.n_group["grav"],
                      n_leaf=sysm.n_leaf, precision=FP64, G=ph.G)
-    g2x, g2y, g2z, pot, etag = tg.with_eta(P["x"], P["y"], P["z"])
-    eta = H * H / (ksrh * ksrh) * omg / (3.0 * dens) * etag
+    g2x, g2y, g2z, pot, _ = tg.with_eta(P["x"], P["y"], P["z"])
+    # constant softening: the potential does not depend on H, so the grad-h
+    # softening term eta of the write-back (hdr_gravity.hpp:110-111) is 0
+    eta = torch.zeros_like(dens)
     # hydro: the device functor and its hand-written VJP, on the EPI
     # preparation of hdr_hydro.hpp:50-60
     hin = {"x": P["x"], "y": P["y"], "z": P["z"], "vx": P["vx"], "vy": P["vy"], "vz": P["vz"],

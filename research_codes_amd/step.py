@@ -14,7 +14,7 @@ are inputs of the step, not part of it.
 The density smoothing-length retry loop (hdr_density.hpp:369-415) is in
 `converge_density`: particles whose H_3 exceeds their search radius get
 rs *= 1.1 and a fresh gather list, and keep the results of the pass in
-which they converged.
+which they converged (their rs then set to 0, as the reference does).
 """
 from __future__ import annotations
 
@@ -34,7 +34,7 @@ KSRH = 1.936492
 
 def phys(**kw) -> L.Phys:
     d = dict(ninv=1.0 / 1.5, ksrh=KSRH, G=1.0, alpha_max=2.0, alpha_min=0.1, alphu_max=0.0,
-             alphu_min=0.0, eps_u=0.0, c_dt=0.1, dt_max=10.0)
+             alphu_min=0.0, eps_u=0.0, c_dt=0.1, dt_max=10.0, eps_const=0.0)
     d.update(kw)
     return L.Phys(**d)
 
@@ -60,6 +60,9 @@ class SPHSystem:
         self.n_leaf = n_leaf
         self.precision = precision
         self.phys = phys_params or phys()
+        # a constant softening leaves the potential independent of H: the
+        # gravity write-back then sets the grad-h softening term eta to 0
+        self.phys.eps_const = 0.0 if eps is None else float(eps)
         # independent systems (an ensemble of runs): particle counts of
         # consecutive blocks of p; each is Morton-sorted and gets its own tree
         n_all = len(p["x"])
@@ -317,20 +320,30 @@ class SPHSystem:
                          self.precision if precision is None else precision)
         return out
 
-    def converge_density(self, expand: float = 1.1, max_rounds: int = 100, jump: bool = True):
-        """calcDensityKernel (hdr_density.hpp:369-415) with host list rebuilds.
+    def converge_density(self, expand: float = 1.1, max_rounds: int = 100, jump: bool = False,
+                         full_lists: bool = False):
+        """calcDensityKernel (hdr_density.hpp:369-415) with device list rebuilds.
 
-        A particle whose H_3 exceeds its search radius retries with
-        rs *= 1.1 (the reference's schedule, jump=False) or with
-        rs = max(1.1 rs, 1.1 H_3) (jump=True, default: the same fixed point in
-        fewer rounds for particles far from their neighbours).  Returns the
-        number of rounds; raises if particles still need a larger search
-        radius after max_rounds (the reference loops without a bound).
+        rs = 1.1 ksr; each round rebuilds the gather lists and runs the density
+        functor.  A particle still retrying (rs != 0) whose H_3 exceeds rs
+        retries with rs *= 1.1; otherwise its results are kept (copyFromForce,
+        hdr_density.hpp:24-31) and its rs set to 0, so later rounds' lists
+        serve only the particles still retrying.  The loop ends when no
+        particle retried.  jump=True grows rs to max(1.1 rs, 1.1 H_3) instead
+        of 1.1 rs (set-up of a star far from the fixed point only: fewer
+        rounds, not the reference's schedule).
+
+        full_lists=True leaves each particle's rs at the value it converged
+        with and, when more than one round ran, rebuilds the gather lists
+        with them, so that the density functor can be re-run on the converged
+        state (the reference leaves rs = 0).  Returns the number of rounds;
+        raises if particles still retry after max_rounds (the reference loops
+        without a bound).
         """
         n = self.n
         s = self.s
         self.rs.copy_(expand * s["ksr"])
-        done = torch.zeros(n, dtype=torch.bool, device=self.dev)
+        rs_conv = torch.zeros_like(self.rs) if full_lists else None
         tmp = {k: torch.empty(n, dtype=torch.float64, device=self.dev)
                for k in ("dens", "divv", "rotv", "grdh", "ksr")}
         tmp["np"] = torch.empty(n, dtype=torch.int64, device=self.dev)
@@ -338,28 +351,37 @@ class SPHSystem:
         for rnd in range(1, max_rounds + 1):
             self.build_lists(("dens",))
             self.density(out=tmp)
-            newly = (~done) & (tmp["itr"] == 0)
+            active = self.rs != 0.0
+            newly = active & (tmp["itr"] == 0)
+            retry = active & ~newly
             for k in ("dens", "divv", "rotv", "grdh", "ksr"):
                 s[k].copy_(torch.where(newly, tmp[k], s[k]))
             self.np_.copy_(torch.where(newly, tmp["np"], self.np_))
-            self.itr.copy_(torch.where(done, self.itr, tmp["itr"]))
-            done |= newly
-            retry = ~done
-            if not bool(retry.any()):
-                return rnd
+            if rs_conv is not None:
+                rs_conv.copy_(torch.where(newly, self.rs, rs_conv))
             grown = self.rs * expand
             if jump:
                 grown = torch.maximum(grown, expand * tmp["ksr"])
-            self.rs.copy_(torch.where(retry, grown, self.rs))
-        raise RuntimeError(f"density did not converge in {max_rounds} rounds")
+            self.rs.copy_(torch.where(retry, grown, torch.where(newly, torch.zeros_like(grown), self.rs)))
+            if not bool(retry.any()):
+                break
+        else:
+            raise RuntimeError(f"density did not converge in {max_rounds} rounds")
+        self.itr.zero_()
+        if full_lists:
+            self.rs.copy_(rs_conv)
+            if rnd > 1:
+                self.build_lists(("dens",))
+        return rnd
 
     def relax_h(self, passes: int = 3) -> int:
         """Run the smoothing-length retry loop `passes` times, each starting
         from the previous H, so H reaches the fixed point of the three-pass
-        iteration (set-up only; the reference starts from relaxed stars)."""
+        iteration (set-up only; the reference starts from relaxed stars, and
+        this uses the jump schedule of converge_density)."""
         rounds = 0
         for _ in range(passes):
-            rounds += self.converge_density()
+            rounds += self.converge_density(jump=True)
         return rounds
 
     def eos_balsara(self):
@@ -467,7 +489,7 @@ class SPHSystem:
     def initialize(self, record: dict | None = None):
         """startSimulation's first force evaluation (hdr_hgas.hpp:1335-1378)."""
         hmax = self.hmax
-        self.converge_density()
+        self.converge_density(full_lists=record is not None)
         self.eos_balsara()
         self.build_lists(("hydro", "grav"))
         self.gravity()
@@ -491,7 +513,7 @@ class SPHSystem:
         check(lib().sphk_predict_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_predict_dev")
         self.resort()
-        self.converge_density()
+        self.converge_density(full_lists=record is not None)
         self.eos_balsara()
         self.build_lists(("hydro", "grav"))
         self.gravity()

@@ -3,7 +3,9 @@
   * EOS: P = (gamma-1) rho u and c^2 = gamma P / rho with gamma = 5/3, which is
     the reference's P = rho u / n, n = 1.5 (hdr_hgas.hpp:339-341,767-768);
   * Balsara switch limits (0 without compression, -> 1 for pure compression);
-  * eta write-back: the closed form eps^2 Omega^-1 / (3 rho) * sum;
+  * gravity write-back: G scaling; eta = 0 under a constant softening (eta
+    with eps = H/kappa is pinned by energy conservation,
+    tests/test_oracle_conservation.py);
   * alpha-dot: relaxes to alpha_min at rate c kappa / (4 H), is pushed up by
     compression only;
   * timestep: power-of-two halving to below the particle minimum; doubling only
@@ -41,13 +43,16 @@ def test_balsara_limits():
     assert abs(s["bswt"][2] - 0.5) < 1e-3
 
 
-def test_gravity_writeback_eta():
+def test_gravity_writeback_scales_by_G_and_zeroes_eta_for_constant_softening():
+    """accg2 and pot are G times the functor's sums.  eta itself is pinned by
+    energy conservation (tests/test_oracle_conservation.py); here only that a
+    constant softening, whose potential does not depend on H, gives eta = 0."""
     s = _state(1, ksr=0.2, grdh=1.1, dens=3.0)
     g = dict(ax=[1.0], ay=[2.0], az=[3.0], pot=[-4.0], eta=[5.0])
     oracle.gravity_writeback(s, oracle.params(G=2.0), g)
-    eps2 = (0.2 / KSRH) ** 2
-    assert abs(s["eta"][0] / (eps2 * 1.1 / (3 * 3.0) * 5.0) - 1) < 1e-14
-    assert s["g2y"][0] == 4.0 and s["pot"][0] == -8.0
+    assert s["g2y"][0] == 4.0 and s["pot"][0] == -8.0 and s["eta"][0] > 0.0
+    oracle.gravity_writeback(s, oracle.params(G=2.0, eps_const=0.01), g)
+    assert s["eta"][0] == 0.0 and s["g2z"][0] == 6.0
 
 
 def _tail(**kw):
