@@ -2,7 +2,7 @@
 
 Workload: off-axis parabolic collision of an n=3 polytrope (M=1, R=1) and an
 n=1.5 polytrope (M=0.5, R=0.6), N = 10^6 equal-mass particles in total,
-impact parameter 0.25 R at separation 3 R, ~50 neighbours (eta = 1.18),
+impact parameter 0.25 R at separation 3 R, ~50 neighbours (eta = 1.25),
 constant softening 0.01, opening angle 0.5, synthetic seeded initial data.
 
 A "step" is one force evaluation with the walk lists resident in HBM: density
@@ -59,7 +59,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--n-group", type=int, default=16)
-    ap.add_argument("--eta", type=float, default=1.18)
+    ap.add_argument("--eta", type=float, default=1.25, help="1.25 realises ~50 neighbours (ic.ETA_50)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="per functor")

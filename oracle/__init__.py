@@ -41,6 +41,35 @@ def build() -> Path:
     return LIB_PATH
 
 
+def build_planted(k: int, out_dir) -> Path:
+    """A variant of the oracle with planted mistake k (sphk_oracle.c,
+    ORC_PLANT), for tests that show a pinning check catches it."""
+    out = Path(out_dir) / f"liboracle_plant{k}.so"
+    subprocess.run(["gcc", "-O2", "-fno-fast-math", "-ffp-contract=off", "-fPIC", "-fopenmp",
+                    "-std=c11", f"-DORC_PLANT={int(k)}", "-shared", "-o", str(out),
+                    str(HERE / "sphk_oracle.c"), "-lm"], check=True)
+    return out
+
+
+class use:
+    """Context manager: route the module's functions through another build
+    of the oracle (e.g. build_planted's) while it is active."""
+
+    def __init__(self, path):
+        self.path = Path(path)
+
+    def __enter__(self):
+        global _lib
+        self._saved = _lib
+        _lib = _configure(C.CDLL(str(self.path)))
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self._saved
+        return False
+
+
 _lib = None
 
 
@@ -49,17 +78,21 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        _lib = C.CDLL(str(LIB_PATH))
-        _lib.orc_kernel0th.restype = C.c_double
-        _lib.orc_kernel0th.argtypes = [C.c_double]
-        _lib.orc_kernel1st.restype = C.c_double
-        _lib.orc_kernel1st.argtypes = [C.c_double]
-        _lib.orc_timestep.restype = C.c_double
-        _lib.orc_energy.restype = C.c_double
-        _lib.orc_ksr_max.restype = C.c_double
-        _lib.orc_unbound_mass.restype = None
-        _lib.orc_unbound_mass.argtypes = [C.c_int, State, _P, C.c_int, C.c_double, C.POINTER(C.c_double)]
+        _lib = _configure(C.CDLL(str(LIB_PATH)))
     return _lib
+
+
+def _configure(d):
+    d.orc_kernel0th.restype = C.c_double
+    d.orc_kernel0th.argtypes = [C.c_double]
+    d.orc_kernel1st.restype = C.c_double
+    d.orc_kernel1st.argtypes = [C.c_double]
+    d.orc_timestep.restype = C.c_double
+    d.orc_energy.restype = C.c_double
+    d.orc_ksr_max.restype = C.c_double
+    d.orc_unbound_mass.restype = None
+    d.orc_unbound_mass.argtypes = [C.c_int, State, _P, C.c_int, C.c_double, C.POINTER(C.c_double)]
+    return d
 
 
 def _ptr(a):

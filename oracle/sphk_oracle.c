@@ -20,6 +20,14 @@
 
 #define JROW(w, k) (j_idx ? j_idx[(k)] : (k))
 
+/* Planted mistakes, for the tests that show that the pinning tests catch a
+ * plausible coefficient error (tests/test_oracle_hydro_terms.py builds
+ * variants with -DORC_PLANT=k).  The default build (k = 0) plants nothing. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(k, bad, good) ((ORC_PLANT == (k)) ? (bad) : (good))
+
 /* hdr_kernel.hpp:246-250 -- W(q) = C0 (1-q)_+^4 (1+4q) */
 double orc_kernel0th(double q)
 {
@@ -182,23 +190,24 @@ void orc_hydro(int n_walk, const int32_t *i_off, const int32_t *j_off, const int
                 double rv = dpx * dvx + dpy * dvy + dpz * dvz;
                 double w_ij = rv * ri;
                 double w0 = ((w_ij < 0.) ? w_ij : 0.);
-                double vs = cs_i + pj.vsnd[j] - 3. * w0;
+                double vs = cs_i + pj.vsnd[j] - PLANT(1, 4., 3.) * w0;
                 double rhi = 1. / (dn_i + pj.dens[j]);
                 double av0 = (bs_i + pj.bswt[j]) * (al_i + pj.alph[j]) * vs * w0;
 
                 vsmx = ((vsmx > vs) ? vsmx : vs);
 
-                double ta = (pc_i + pj.presc[j] - 0.5 * av0 * rhi) * ka * ri;
+                double ta = (pc_i + pj.presc[j] - PLANT(2, 1., 0.5) * av0 * rhi) * ka * ri;
                 achx -= ta * dpx;
                 achy -= ta * dpy;
                 achz -= ta * dpz;
 
-                double vsu2 = fabs(pp_i - pj.pres[j]) * rhi * 2.;
+                double vsu2 = fabs(pp_i - pj.pres[j]) * rhi * PLANT(3, 1., 2.);
                 double vsui = ((vsu2 != 0.) ? 1. / sqrt(vsu2) : 0.);
                 double vsu = vsu2 * vsui;
                 double du = eg_i - pj.thrm[j];
 
-                udot += ka * (pc_i * w_ij - rhi * (0.25 * av0 * w0 - (au_i + pj.alphu[j]) * vsu * du));
+                udot += ka * (pc_i * w_ij - rhi * (PLANT(4, 0.5, 0.25) * av0 * w0
+                                                     - PLANT(8, au_i, au_i + pj.alphu[j]) * vsu * du));
                 difu += rhi * du * ka * ri;
 
                 double dg = pj.m[j] * ri * (et_i * dw_i + pj.eta[j] * dw_j);
@@ -345,7 +354,7 @@ void orc_hydro_writeback_tail(int n, orc_state s, orc_params p,
         double lim = p.alphu_max - s.alphu[i];
         lim = (lim > 0.) ? lim : 0.;
         double srcu = s.ksr[i] * ksrhinv * fabs(s.diffu[i]) / sqrt(uene + p.eps_u) * lim;
-        s.adotu[i] = -(s.alphu[i] - p.alphu_min) * tauinv + srcu;
+        s.adotu[i] = -(s.alphu[i] - p.alphu_min) * tauinv + PLANT(5, src, srcu);
     }
 }
 
@@ -356,9 +365,10 @@ double orc_timestep(int n, orc_state s, orc_params p, double dt_prev, double tim
         /* hdr_hgas.hpp:477-488 */
         double dth = p.c_dt * s.ksr[i] / (s.vsmx[i] * p.ksrh);
         double dtu = p.dt_max;
-        double dtcv = (s.divv[i] < 0.) ? (-p.c_dt / s.divv[i]) : p.dt_max;
+        double dtcv = (s.divv[i] < 0.) ? PLANT(6, -p.c_dt * s.divv[i], -p.c_dt / s.divv[i]) : p.dt_max;
         double gx = s.g1x[i] + s.g2x[i], gy = s.g1y[i] + s.g2y[i], gz = s.g1z[i] + s.g2z[i];
-        double dtg = p.c_dt * sqrt(s.ksr[i] / sqrt(gx * gx + gy * gy + gz * gz));
+        double dtg = p.c_dt * PLANT(7, s.ksr[i] / sqrt(gx * gx + gy * gy + gz * gz),
+                                      sqrt(s.ksr[i] / sqrt(gx * gx + gy * gy + gz * gz)));
         double a = (dth < dtu) ? dth : dtu;
         double b = (dtg < dtcv) ? dtg : dtcv;
         double d = (a < b) ? a : b;

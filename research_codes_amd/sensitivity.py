@@ -17,7 +17,7 @@ from .step import SPHSystem
 
 
 def head_on_loss(dx0: float = 0.0, dv0: float = 0.0, n_each: int = 100_000, steps: int = 200,
-                 dt: float = 1e-3, k: float = 20.0, eta: float = 1.18, eps: float = 0.01,
+                 dt: float = 1e-3, k: float = 20.0, eta: float = ic.ETA_50, eps: float = 0.01,
                  precision: int = MIXED, theta: float = 0.5, separation: float = 3.0,
                  dts=None) -> float:
     """Smoothed unbound mass after `steps` KDK steps of length dt (or the

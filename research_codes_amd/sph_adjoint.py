@@ -512,7 +512,7 @@ class CollisionRun:
 
 
 def head_on_gradient(n_each: int = 100_000, steps: int = 200, dt: float | None = 1e-3, k: float = 20.0,
-                     eta: float = 1.18, eps: float = 0.01, theta: float = 0.5, checkpoint_every: int = 20,
+                     eta: float = ic.ETA_50, eps: float = 0.01, theta: float = 0.5, checkpoint_every: int = 20,
                      n_group: int = 32, separation: float = 3.0) -> tuple[float, np.ndarray, list]:
     """Smoothed unbound mass after `steps` KDK steps of the head-on
     collision of two n=1.5 stars (research_codes_amd.sensitivity.head_on_loss

@@ -1,8 +1,10 @@
 """Size-independent properties of one force step at the full benchmark size.
 
 N = 10^6 off-axis collision of an n=3 (M=1, R=1) and an n=1.5 (M=0.5,
-R=0.6) polytrope, eta = 1.18, constant softening 0.01, theta = 0.5, lists
-from the device tree (the bench.py workload).  At this size the oracle is
+R=0.6) polytrope, eta = 1.25 (~50 neighbours), constant softening 0.01,
+theta = 0.5, lists from the device tree (the bench.py workload), with each
+star given a rotation about its centre of mass plus a random velocity and
+alpha_u = 0.5, so that the viscosity, conduction and du/dt terms are live.  At this size the oracle is
 too slow to run whole, so the step is checked through what the mathematics
 fixes (hdr_density.hpp:288-299, hdr_hydro.hpp:366-409):
 
@@ -19,16 +21,25 @@ import torch
 
 from research_codes_amd import ic
 from research_codes_amd.device import FP64, MIXED, Context
-from research_codes_amd.step import KSRH, SPHSystem
+from research_codes_amd.step import KSRH, SPHSystem, phys
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
 def system():
-    p = ic.off_axis(1_000_000, eta=1.18)
+    p = ic.off_axis(1_000_000, eta=ic.ETA_50)
+    rng = np.random.default_rng(17)
+    for k in (0, 1):
+        sel = p["istar"] == k
+        cx, cy = (np.average(p[c][sel], weights=p["m"][sel]) for c in ("x", "y"))
+        p["vx"][sel] += -0.2 * (p["y"][sel] - cy) + 0.02 * rng.standard_normal(sel.sum())
+        p["vy"][sel] += 0.2 * (p["x"][sel] - cx) + 0.02 * rng.standard_normal(sel.sum())
+        p["vz"][sel] += 0.02 * rng.standard_normal(sel.sum())
+    p["alphu"] = np.full(len(p["m"]), 0.5)
     with Context(0) as c:
-        s = SPHSystem(p, c, eta=1.18, eps=0.01, theta=0.5, n_group=32, precision=FP64)
+        s = SPHSystem(p, c, eta=ic.ETA_50, eps=0.01, theta=0.5, n_group=32, precision=FP64,
+                      phys_params=phys(alphu_max=1.0, eps_u=1e-3))
         s.relax_h(2)
         s.eos_balsara()
         s.rs.copy_(1.1 * s.s["ksr"])
@@ -69,9 +80,13 @@ def test_full_size_step_properties(system):
     e_w = float(work.sum().abs() / w_scale)
     print(f"[full] hydro momentum {e_p:.2e}, energy {e_w:.2e} (relative to the sums of magnitudes)")
     assert e_p < 1e-10 and e_w < 1e-10
+    # the dissipative terms are live
+    for k in ("divv", "rotv", "bswt"):
+        assert float(st[k].abs().max()) > 0.0, k
+    assert float(h["udot"].abs().max()) > 0.0 and float(h["diffu"].abs().max()) > 0.0
     npi = s.np_.double().mean().item()
     print(f"[full] mean neighbour count {npi:.1f}")
-    assert 30.0 < npi < 70.0       # ~40 at eta = 1.18 (4 pi / 3 (eta kappa)^3 = 50 in a uniform medium)
+    assert 47.0 < npi < 53.0       # ~50 at eta = 1.25 (ic.ETA_50)
 
 
 def test_full_size_mixed_matches_fp64(system):
