@@ -96,6 +96,16 @@ class LaneEmden:
         self.mu1 = self.mu[-1]                         # = -xi1^2 theta'(xi1)
 
 
+def structure(le: LaneEmden, M: float, R: float, G: float = 1.0) -> tuple[float, float]:
+    """Central density and polytropic constant of the n-polytrope of mass M,
+    radius R: rho_c = M xi_1 / (4 pi R^3 |theta'(xi_1)|),
+    K = 4 pi G (R / xi_1)^2 rho_c^(1 - 1/n) / (n + 1)."""
+    n = le.n
+    rho_c = M * le.xi1 / (4.0 * math.pi * R ** 3 * (-le.dtheta1))
+    K = 4.0 * math.pi * G * (R / le.xi1) ** 2 * rho_c ** (1.0 - 1.0 / n) / (n + 1.0)
+    return rho_c, K
+
+
 def polytrope(n_index: float, N: int, M: float = 1.0, R: float = 1.0, seed: int = 0,
               eta: float = 1.6, id0: int = 0, istar: int = 0, le: LaneEmden | None = None) -> dict:
     """N equal-mass particles sampled from an n-polytrope of mass M, radius R."""
@@ -110,12 +120,10 @@ def polytrope(n_index: float, N: int, M: float = 1.0, R: float = 1.0, seed: int 
     x = r * sth * np.cos(phi)
     y = r * sth * np.sin(phi)
     z = r * cth
-    rho_c = M * le.xi1 / (4.0 * math.pi * R ** 3 * (-le.dtheta1))
+    rho_c, K = structure(le, M, R)
     th = np.interp(xi, le.xi, le.theta)
     rho = rho_c * np.maximum(th, 0.0) ** n_index
     rho = np.maximum(rho, rho_c * 1e-4)               # surface particles: avoid rho = 0
-    alpha_len = R / le.xi1
-    K = 4.0 * math.pi * alpha_len ** 2 * rho_c ** (1.0 - 1.0 / n_index) / (n_index + 1.0)
     P = K * rho ** (1.0 + 1.0 / n_index)
     uene = P / ((GAMMA - 1.0) * rho)
     m = np.full(N, M / N)
@@ -148,11 +156,16 @@ def collision(star0: dict, star1: dict, separation: float = 3.0, impact: float =
     v = np.array([-math.sqrt(2.0 * G * M / separation), 0.0, 0.0])
     s0 = dict(star0)
     s1 = dict(star1)
+    # each star's own (sampled) centre of mass and bulk velocity are removed
+    # first, so the separation and speed are those of the two centres
+    for s, star in ((s0, star0), (s1, star1)):
+        for c in ("x", "y", "z", "vx", "vy", "vz"):
+            s[c] = star[c] - np.average(star[c], weights=star["m"])
     for k, c in enumerate("xyz"):
-        s0[c] = star0[c] - M1 / M * d[k]
-        s1[c] = star1[c] + M0 / M * d[k]
-        s0["v" + c] = star0["v" + c] - M1 / M * v[k]
-        s1["v" + c] = star1["v" + c] + M0 / M * v[k]
+        s0[c] = s0[c] - M1 / M * d[k]
+        s1[c] = s1[c] + M0 / M * d[k]
+        s0["v" + c] = s0["v" + c] - M1 / M * v[k]
+        s1["v" + c] = s1["v" + c] + M0 / M * v[k]
     s1["id"] = star1["id"] + len(star0["id"])
     s1["istar"] = np.ones_like(star1["istar"])
     return concat(s0, s1)
