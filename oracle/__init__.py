@@ -87,6 +87,9 @@ def _configure(d):
     d.orc_kernel0th.argtypes = [C.c_double]
     d.orc_kernel1st.restype = C.c_double
     d.orc_kernel1st.argtypes = [C.c_double]
+    for f in (d.orc_rcp_4th, d.orc_rsqrt_4th):
+        f.restype = C.c_double
+        f.argtypes = [C.c_double]
     d.orc_timestep.restype = C.c_double
     d.orc_energy.restype = C.c_double
     d.orc_ksr_max.restype = C.c_double
@@ -167,8 +170,9 @@ def _walk_args(i_off, j_off, j_idx):
 
 
 def density(i_off, j_off, j_idx, pi: dict, pj: dict, eta=1.6, ksrh=1.936492, hmax=1e30,
-            nthreads=0) -> dict:
-    """Scalar calcDensity over walks.  pi/pj: dicts of SoA arrays."""
+            nthreads=0, avx2=False) -> dict:
+    """Scalar calcDensity over walks.  pi/pj: dicts of SoA arrays.  avx2=True:
+    the AVX2 branch's arithmetic (approximate reciprocals) instead."""
     i_off, j_off, j_idx = _walk_args(i_off, j_off, j_idx)
     keep = []
 
@@ -183,7 +187,8 @@ def density(i_off, j_off, j_idx, pi: dict, pj: dict, eta=1.6, ksrh=1.936492, hma
     out["np"] = np.zeros(ni, np.int64)
     out["itr"] = np.zeros(ni, np.uint8)
     o = DensOut(**{k: _ptr(v) for k, v in out.items()})
-    lib().orc_density(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pi, True),
+    fn = lib().orc_density_avx2 if avx2 else lib().orc_density
+    fn(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pi, True),
                       src(pj, False), C.c_double(eta), C.c_double(ksrh), C.c_double(hmax), o,
                       int(nthreads))
     return out

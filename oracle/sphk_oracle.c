@@ -12,6 +12,7 @@
  */
 #include "sphk_oracle.h"
 
+#include <immintrin.h>
 #include <math.h>
 #include <stddef.h>
 #ifdef _OPENMP
@@ -55,10 +56,36 @@ static void set_threads(int nthreads)
 #endif
 }
 
-/* Scalar calcDensity, hdr_density.hpp:254-363. */
-void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
-                 orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
-                 orc_dens_out out, int nthreads)
+/* The AVX2 build's reciprocals (vector_x86.hpp:492-534, used by the
+ * USE_INTRINSICS calcDensity, hdr_density.hpp:89-91): a single-precision
+ * hardware estimate (about 12 bits; the estimate tables differ between CPU
+ * vendors) refined in FP64 by the 4th-order series of 1/x and 1/sqrt(x). */
+static double est_rcp(double x) { return (double)_mm_cvtss_f32(_mm_rcp_ss(_mm_set_ss((float)x))); }
+static double est_rsqrt(double x) { return (double)_mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss((float)x))); }
+
+double orc_rcp_4th(double x)
+{
+    double x0 = est_rcp(x);
+    double h = 1. - x * x0;
+    return (1. + h) * (1. + h * h) * x0;
+}
+
+double orc_rsqrt_4th(double x)
+{
+    double x0 = est_rsqrt(x);
+    double h = 1. - x * x0 * x0;
+    return x0 + (0.5 + (0.375 + (0.3125 + 0.2734375 * h) * h) * h) * h * x0;
+}
+
+static double rcp_exact(double x) { return 1. / x; }
+static double rsqrt_exact(double x) { return 1. / sqrt(x); }
+
+/* Scalar calcDensity, hdr_density.hpp:254-363 (rcp, rsqrt exact), or the
+ * arithmetic of the AVX2 branch, hdr_density.hpp:82-242 (rcp_4th, rsqrt_4th
+ * in place of 1/x and 1/sqrt(x); everything else as the scalar one). */
+static void density_impl(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                         orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
+                         orc_dens_out out, int nthreads, double (*rcp)(double), double (*rsqrt)(double))
 {
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -72,7 +99,7 @@ void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const i
 
             /* three fixed-point passes, hdr_density.hpp:264-302 */
             for (int repeat = 0; repeat < 3; repeat++) {
-                double hi_i = 1. / hs_i;
+                double hi_i = rcp(hs_i);
                 double hi3_i = hi_i * hi_i * hi_i; /* hdr_dimension.hpp:121-123 */
                 double rh_i = 0.;
                 int64_t nj_i = 0;
@@ -100,7 +127,7 @@ void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const i
             }
 
             /* grad-h / divergence / curl pass, hdr_density.hpp:304-349 */
-            double hi_i = 1. / hs_i;
+            double hi_i = rcp(hs_i);
             double hi4_i = hi_i * (hi_i * hi_i * hi_i);
             double grdh_i = 0., divv_i = 0., rotx_i = 0., roty_i = 0., rotz_i = 0.;
             for (int k = jb; k < je; k++) {
@@ -112,8 +139,7 @@ void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const i
                 double dvy = vy_i - pj.vy[j];
                 double dvz = vz_i - pj.vz[j];
                 double r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-                double ri = 1. / sqrt(r2);
-                ri = ((id_i != (int64_t)pj.id[j]) ? ri : 0.);
+                double ri = ((id_i != (int64_t)pj.id[j]) ? rsqrt(r2) : 0.);
                 double r1 = r2 * ri;
                 double q = r1 * hi_i;
                 double kw0 = orc_kernel0th(q);
@@ -136,10 +162,11 @@ void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const i
             }
             /* hdr_density.hpp:351-362 (no NaN guard on Omega, as in the CPU functor) */
             double dens_i = out.dens[i];
-            double deni_i = 1. / dens_i;
-            double omgi_i = 1. / (1. + hs_i * deni_i * grdh_i / 3.);
+            double deni_i = rcp(dens_i);
+            double omgi_i = (rcp == rcp_exact) ? 1. / (1. + hs_i * deni_i * grdh_i / 3.)
+                                               : rcp(1. + hs_i * deni_i * grdh_i * rcp(3.));
             double rot2 = rotx_i * rotx_i + roty_i * roty_i + rotz_i * rotz_i;
-            double rotv_i = rot2 * ((rot2 != 0.) ? 1. / sqrt(rot2) : 0.);
+            double rotv_i = rot2 * ((rot2 != 0.) ? rsqrt(rot2) : 0.);
             rotv_i *= deni_i * omgi_i;
             divv_i *= deni_i * omgi_i;
             out.divv[i] = divv_i;
@@ -147,6 +174,21 @@ void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const i
             out.grdh[i] = omgi_i;
         }
     }
+}
+
+void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                 orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
+                 orc_dens_out out, int nthreads)
+{
+    density_impl(n_walk, i_off, j_off, j_idx, pi, pj, eta, ksrh, hmax, out, nthreads, rcp_exact, rsqrt_exact);
+}
+
+void orc_density_avx2(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                      orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
+                      orc_dens_out out, int nthreads)
+{
+    density_impl(n_walk, i_off, j_off, j_idx, pi, pj, eta, ksrh, hmax, out, nthreads, orc_rcp_4th,
+                 orc_rsqrt_4th);
 }
 
 /* Scalar calcHydro, hdr_hydro.hpp:320-421. */

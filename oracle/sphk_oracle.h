@@ -78,6 +78,17 @@ double orc_kernel1st(double q);
 void orc_density(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
                  orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
                  orc_dens_out out, int nthreads);
+/* The AVX2 branch's arithmetic (hdr_density.hpp:82-242): orc_density with
+ * the approximate reciprocals below in place of 1/x and 1/sqrt(x).  Its
+ * seeds come from the host CPU's estimate instructions, so its last bits are
+ * the host's; it quantifies the shipped CPU path's difference from the
+ * scalar functor.  The tail read past nip of that branch is not reproduced. */
+void orc_density_avx2(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                      orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
+                      orc_dens_out out, int nthreads);
+/* rcp_4th / rsqrt_4th of vector_x86.hpp:501-534, one lane. */
+double orc_rcp_4th(double x);
+double orc_rsqrt_4th(double x);
 
 /* Scalar calcHydro, hdr_hydro.hpp:312-424.  Outputs are assigned. */
 void orc_hydro(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
