@@ -164,6 +164,75 @@ def expand_pairs(w: Walks) -> tuple[torch.Tensor, torch.Tensor]:
     return i, j
 
 
+# --------------------------------------------------------- frozen gravity
+class FrozenGravity:
+    """Tree gravity over a recorded tree and long lists (one KDK step's, from
+    SPHSystem.kdk(record=...)): the node membership, the EPJ/SPJ lists and
+    the node masses stay those of the record; the nodes' centres of mass are
+    recomputed from the positions it is called with, X_n = sum_{k in n} m_k
+    x_k / M_n.  This is the tree held fixed, the piecewise-constant branch of
+    tree gravity the adjoint differentiates, as a function that can also be
+    evaluated away from the recorded state (finite differences at fixed
+    topology).  Uniform softening eps2; outputs a and pot times G."""
+
+    def __init__(self, ctx: Context, m, eps2: float, walks: Walks, begin, end, G: float):
+        self.ctx, self.m, self.e2, self.walks, self.G = ctx, m.contiguous(), float(eps2), walks, G
+        lengths = (end - begin).long()
+        n_node = lengths.numel()
+        node_of = torch.repeat_interleave(torch.arange(n_node, device=m.device), lengths)
+        start = torch.cumsum(lengths, 0) - lengths
+        self.idx = begin.long()[node_of] + (torch.arange(node_of.numel(), device=m.device) - start[node_of])
+        self.node_of, self.lengths = node_of, lengths
+        self.M = torch.segment_reduce(self.m[self.idx], "sum", lengths=lengths, unsafe=True, initial=0.0)
+        self.e2n = torch.full_like(self.M, self.e2)
+        self.e2p = torch.full_like(self.m, self.e2)
+
+    def centres(self, x, y, z):
+        return [_SegSum.apply(_Gather.apply(self.m * c, self.idx), self.node_of, self.lengths) / self.M
+                for c in (x, y, z)]
+
+    def __call__(self, x, y, z):
+        X, Y, Z = self.centres(x, y, z)
+        out = _FrozenGravityFn.apply(x, y, z, X, Y, Z, self)
+        return tuple(o * self.G for o in out)
+
+
+class _FrozenGravityFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, y, z, X, Y, Z, op):
+        x, y, z, X, Y, Z = (t.contiguous() for t in (x, y, z, X, Y, Z))
+        src = dict(m=op.m, x=x, y=y, z=z, eps2=op.e2p)
+        ps = dict(m=op.M, x=X, y=Y, z=Z, eps2=op.e2n)
+        out = {k: torch.empty_like(op.m) for k in L.GRAV_OUT}
+        keep = op.ctx.params["uniform_softening"]
+        op.ctx.set_params(uniform_softening=True)
+        op.ctx.gravity(op.walks, src, src, out, ps=ps, precision=FP64)
+        op.ctx.set_params(uniform_softening=keep)
+        ctx.op = op
+        ctx.save_for_backward(x, y, z, X, Y, Z)
+        return out["ax"], out["ay"], out["az"], out["pot"]
+
+    @staticmethod
+    def backward(ctx, gax, gay, gaz, gpot):
+        op = ctx.op
+        x, y, z, X, Y, Z = ctx.saved_tensors
+        w = op.walks
+        g = [torch.zeros_like(x) for _ in range(3)]
+        nb = [torch.zeros_like(X) for _ in range(3)]
+        src = L.GravSrc(m=op.m.data_ptr(), x=x.data_ptr(), y=y.data_ptr(), z=z.data_ptr(),
+                        eps2=op.e2p.data_ptr())
+        ps = L.GravSrc(m=op.M.data_ptr(), x=X.data_ptr(), y=Y.data_ptr(), z=Z.data_ptr(),
+                       eps2=op.e2n.data_ptr())
+        z0 = torch.zeros_like(x)
+        cot = [(z0 if c is None else c).contiguous() for c in (gax, gay, gaz, gpot)]
+        L.check(L.lib().sphk_gravity_vjp_dev(op.ctx.handle, w.n_walk, w.i_off.data_ptr(), w.j_off.data_ptr(),
+                                             w.j_idx.data_ptr(), src, w.s_off.data_ptr(), w.s_idx.data_ptr(),
+                                             ps, src, op.e2, *(c.data_ptr() for c in cot), None,
+                                             *(a.data_ptr() for a in g), *(a.data_ptr() for a in nb),
+                                             torch.cuda.current_stream().cuda_stream), "sphk_gravity_vjp_dev")
+        return (*g, *nb, None)
+
+
 # ------------------------------------------------------------------ force
 def _density_t(P, m, pairs, hmax, eta, ksrh):
     """The density functor on pairs (i, j): returns dens, ksr (= H_3), divv,
@@ -331,10 +400,16 @@ def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
     vsnd = _sqrt0((1.0 + ph.ninv) * pres / dens)
     ad = torch.abs(divv)
     bswt = ad / (ad + torch.abs(rotv) + 1e-4 * vsnd * ksrh / H)
-    # gravity (uniform softening) and its write-back
-    tg = TreeGravity(sysm.ctx, m.detach(), eps=sysm.eps, theta=sysm.theta, n_group=sysm.n_group["grav"],
-                     n_leaf=sysm.n_leaf, precision=FP64, G=ph.G)
-    g2x, g2y, g2z, pot, _ = tg.with_eta(P["x"], P["y"], P["z"])
+    # gravity (uniform softening) and its write-back: over the recorded tree
+    # and long lists when the record holds them, else a tree of its own
+    if "grav" in rec:
+        fg = FrozenGravity(sysm.ctx, m.detach(), sysm.eps ** 2, rec["grav"], rec["gnode_begin"],
+                           rec["gnode_end"], ph.G)
+        g2x, g2y, g2z, pot = fg(P["x"], P["y"], P["z"])
+    else:
+        tg = TreeGravity(sysm.ctx, m.detach(), eps=sysm.eps, theta=sysm.theta, n_group=sysm.n_group["grav"],
+                         n_leaf=sysm.n_leaf, precision=FP64, G=ph.G)
+        g2x, g2y, g2z, pot, _ = tg.with_eta(P["x"], P["y"], P["z"])
     # constant softening: the potential does not depend on H, so the grad-h
     # softening term eta of the write-back (hdr_gravity.hpp:110-111) is 0
     eta = torch.zeros_like(dens)
@@ -412,6 +487,8 @@ def smoothed_unbound_t(S: dict, m, k: float):
 
 # ------------------------------------------------------------ checkpoints
 _EXTRA = ("id", "istar", "rs", "np_", "itr", "eps2")
+# the initial state's differentiable fields
+_INIT = ("x", "y", "z", "vx", "vy", "vz", "uene", "alph", "alphu", "ksr")
 
 
 def _save(sysm: SPHSystem) -> dict:
@@ -452,14 +529,17 @@ class CollisionRun:
     holds fixed (kept in `dts`).  `backward()` returns the gradient of the
     loss with respect to the initial state (a dict of per-particle tensors
     per field, in the order the system holds its particles at construction).
-    The system must be in FP64 precision with a uniform gravitational
-    softening."""
+    The system needs a uniform gravitational softening.  Its precision is
+    that of the forward run and of the backward pass's replay (SPHSystem
+    precision; MIXED: FP32 pair terms, FP64 accumulation); the restated step
+    the gradient is taken through is FP64, linearised at the run's states.
+    `frozen_losses(deltas)` evaluates the loss with every discrete choice of
+    this run held (tree, lists, reorders, H_max, step lengths) for perturbed
+    initial states: the function whose derivative `backward()` is."""
 
     def __init__(self, sysm: SPHSystem, dt: float | None, checkpoint_every: int = 20, k: float = 20.0):
         if sysm.eps is None:
             raise ValueError("the gravity adjoint needs a uniform softening (eps)")
-        if sysm.precision != FP64:
-            raise ValueError("the restated step is FP64: run the system in FP64 precision")
         if len(sysm.seg) != 2:
             raise ValueError("one run per system: differentiate the runs of an ensemble one by one")
         if sysm.tree != "device":
@@ -505,10 +585,33 @@ class CollisionRun:
                 out = step_t(inp, m, rec, s, dt)
                 cot = _pull(out, inp, cot)
                 del out, inp
-        inp = {k: self.init_state[k].clone().requires_grad_(True)
-               for k in ("x", "y", "z", "vx", "vy", "vz", "uene", "alph", "alphu", "ksr")}
+        inp = {k: self.init_state[k].clone().requires_grad_(True) for k in _INIT}
         out = init_t(inp, self.m0, self.init_rec, s)
         return _pull(out, inp, cot)
+
+    @torch.no_grad()
+    def frozen_losses(self, deltas: list) -> list:
+        """The loss after the run's steps, with this run's tree, lists,
+        reorders, H_max and step lengths held (re-recorded by replaying its
+        HIP steps segment by segment), for initial states perturbed by each
+        delta (a dict of per-particle tensors over the _INIT fields, in the
+        construction order).  The perturbed states go through the restated
+        FP64 step."""
+        s = self.sysm
+        states = []
+        for d in deltas:
+            S0 = {k: self.init_state[k] + d[k] if k in d else self.init_state[k] for k in _INIT}
+            f = init_t(S0, self.m0, self.init_rec, s)
+            states.append({k: f[k] for k in STATE})
+        for n0 in sorted(self.cks):
+            n1 = min(n0 + self.every, self.steps)
+            _restore(s, self.cks[n0])
+            for n in range(n0, n1):
+                m, rec = s.s["m"].clone(), {}
+                s.kdk(dt=self.dts[n], record=rec)
+                states = [{k: v for k, v in step_t(S, m, rec, s, self.dts[n]).items() if k in STATE}
+                          for S in states]
+        return [float(smoothed_unbound_t(S, self.m_final, self.k)) for S in states]
 
 
 def head_on_gradient(n_each: int = 100_000, steps: int = 200, dt: float | None = 1e-3, k: float = 20.0,

@@ -496,8 +496,19 @@ class SPHSystem:
         self.hydro()
         if record is not None:
             record.update(perm=None, hmax=hmax, dens=self.lists["dens"], hydro=self.lists["hydro"],
-                          id=self.id.clone())
+                          id=self.id.clone(), **self._grav_record())
         self.update_hmax()
+
+    def _grav_record(self) -> dict:
+        """The gravity step's long lists and its tree's node ranges (one run;
+        an ensemble records nothing here)."""
+        if len(self.seg) != 2 or self.tree != "device":
+            return {}
+        t = self._dtrees.get((0, self.n, self.n_group["grav"]))
+        if t is None:
+            return {}
+        nd = t.nodes()
+        return dict(grav=self.lists["grav"], gnode_begin=nd["begin"], gnode_end=nd["end"])
 
     def kdk(self, dt: float | None = None, record: dict | None = None):
         """One loopSimulation step (hdr_hgas.hpp:1499-1539) with a fixed list
@@ -520,7 +531,7 @@ class SPHSystem:
         self.hydro()
         if record is not None:
             record.update(perm=self.last_perm, hmax=hmax, dens=self.lists["dens"],
-                          hydro=self.lists["hydro"], id=self.id.clone())
+                          hydro=self.lists["hydro"], id=self.id.clone(), **self._grav_record())
         self.update_hmax()
         check(lib().sphk_correct_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_correct_dev")

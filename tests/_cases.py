@@ -84,3 +84,70 @@ def scaled_err(got, ref, floor_frac=1e-3):
     rms = float(np.sqrt(np.mean(ref * ref))) if ref.size else 0.0
     den = np.maximum(np.abs(ref), floor_frac * rms + 1e-300)
     return float(np.max(np.abs(got - ref) / den)) if ref.size else 0.0
+
+
+def check_walks_against_oracle(sysm, precision, n_blocks=4, per_block=16, eps=None):
+    """Run the three functors of SPHSystem `sysm` on its current lists and
+    inputs (into scratch outputs, the state untouched) and compare them with
+    the oracle on `n_blocks` evenly spaced blocks of `per_block` walks each.
+    Returns {field: max |gpu - oracle| / max |oracle|} and the number of
+    neighbour-count mismatches."""
+    import torch
+
+    W = sysm.lists
+    s = {k: v.cpu().numpy() for k, v in sysm.s.items()}
+    ids = sysm.id.cpu().numpy()
+    rs = sysm.rs.cpu().numpy()
+    hp = {k: v.cpu().numpy() for k, v in sysm.hprep.items()}
+    eps2 = sysm.eps2.cpu().numpy()
+    mom = {k: v.cpu().numpy() for k, v in sysm.mom.items()}
+    gdev = dict(m=sysm.s["m"], x=sysm.s["x"], y=sysm.s["y"], z=sysm.s["z"], eps2=sysm.eps2)
+    od = {k: torch.empty_like(sysm.s["m"]) for k in ("dens", "divv", "rotv", "grdh", "ksr")}
+    od["np"] = torch.empty_like(sysm.np_)
+    od["itr"] = torch.empty_like(sysm.itr)
+    sysm.ctx.density(W["dens"], sysm.dens_src(), sysm.dens_src(), od, precision)
+    oh = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx", "diffu")}
+    sysm.ctx.hydro(W["hydro"], sysm.hydro_src(), sysm.hydro_src(), oh, precision)
+    og = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "pot", "eta")}
+    sysm.ctx.gravity(W["grav"], gdev, gdev, og, ps=sysm.mom, precision=precision)
+    torch.cuda.synchronize()
+    gpu = {"dens": {k: v.cpu().numpy() for k, v in od.items()},
+           "hydro": {k: v.cpu().numpy() for k, v in oh.items()},
+           "grav": {k: v.cpu().numpy() for k, v in og.items()}}
+    dsrc = dict(id=ids, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"], vz=s["vz"],
+                ksr=s["ksr"], rs=rs)
+    hsrc = dict(id=ids, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"], vz=s["vz"],
+                hinv=hp["hinv"], pres=s["pres"], presc=hp["presc"], bswt=hp["bswt_half"],
+                dens=s["dens"], vsnd=s["vsnd"], alph=s["alph"], alphu=s["alphu"], thrm=hp["thrm"],
+                eta=s["eta"])
+    gsrc = dict(m=s["m"], x=s["x"], y=s["y"], z=s["z"], eps2=eps2)
+    errs, np_mis = {}, 0
+    for kind in ("dens", "hydro", "grav"):
+        w = W[kind]
+        i_off, j_off = w.i_off.cpu().numpy(), w.j_off.cpu().numpy()
+        j_idx = w.j_idx.cpu().numpy()
+        s_off = None if w.s_off is None else w.s_off.cpu().numpy()
+        s_idx = None if w.s_idx is None else w.s_idx.cpu().numpy()
+        nw = len(i_off) - 1
+        for b in range(n_blocks):
+            w0 = b * nw // n_blocks
+            sl = slice(w0, min(nw, w0 + per_block) + 1)
+            io, jo = i_off[sl], j_off[sl]
+            if kind == "dens":
+                o = oracle.density(io, jo, j_idx, dsrc, dsrc, eta=sysm.eta, hmax=sysm.ctx.params["hmax"])
+            elif kind == "hydro":
+                o = oracle.hydro(io, jo, j_idx, hsrc, hsrc)
+            else:
+                o = oracle.gravity(io, jo, j_idx, gsrc, gsrc, s_off=s_off[sl], s_idx=s_idx, ps=mom)
+            ix = np.arange(int(io[0]), int(io[-1]))
+            for k, r in o.items():
+                if k == "itr":
+                    continue
+                g = gpu[kind][k][ix]
+                r = r[ix]
+                if k == "np":
+                    np_mis += int((g != r).sum())
+                    continue
+                e = float(np.abs(g - r).max() / max(np.abs(r).max(), 1e-300)) if len(r) else 0.0
+                errs[f"{kind}.{k}"] = max(errs.get(f"{kind}.{k}", 0.0), e)
+    return errs, np_mis
