@@ -95,6 +95,20 @@ int sphk_correct_dev(sphk_ctx *ctx, int n, sphk_state s, double dt, void *stream
  *   out[5] total mass */
 int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *out, void *stream);
 
+/* Allocation-free variant of sphk_diagnostics.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels, grid and reduction order as sphk_diagnostics (bitwise the same
+ * six values), but the results go to DEVICE memory out_dev[0..5],
+ * asynchronously on `stream` with no synchronisation, and the 1024 per-block
+ * partials live in a file-scope device array of 6 x 1024 doubles (48 KiB) that
+ * each GPU holds until the process ends.  No hipMalloc, hipFree or allocating
+ * library call is made.  Any n >= 0 is accepted (the kernels stride over the
+ * particles with the fixed grid, so the scratch does not grow with n).  Calls
+ * that overlap on one GPU share that array and race: serialise them. */
+int sphk_diagnostics_static(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *out_dev,
+                            void *stream);
+
 /* Unbound mass of the current state (analysis/unbound_mass/unbound_mass.py:10-28,
  * analysis/common.py:71-90): v_cg = sum rho v / sum rho over the selected
  * particles, e_i = |v_i - v_cg|^2 / 2 + pot_i + u_i; hard = sum of m_i with

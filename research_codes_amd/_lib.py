@@ -106,6 +106,7 @@ SIGNATURES = {
     "sphk_predict_dev": (C.c_int, [_P, C.c_int, State, C.c_double, _P]),
     "sphk_correct_dev": (C.c_int, [_P, C.c_int, State, C.c_double, _P]),
     "sphk_diagnostics": (C.c_int, [_P, C.c_int, State, Phys, _DP, _P]),
+    "sphk_diagnostics_static": (C.c_int, [_P, C.c_int, State, Phys, _P, _P]),
     # sphk_tree.h
     "sphk_morton_order": (C.c_int, [C.c_int, _P, _P, _P, _P]),
     "sphk_tree_build": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(_P)]),

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "sphk.h"
 #include "sphk_internal.hpp"
@@ -320,6 +321,38 @@ __global__ __launch_bounds__(kBlock) void sum_final_kernel(int nb, const double 
         for (int k = 0; k < 6; k++) out[k] = acc[k];
 }
 
+// ---- allocation-free diagnostics ------------------------------------------
+// The partials of sphk_diagnostics_static: 6 sums per block of the fixed
+// 1024-block grid (48 KiB).  Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for that GPU (kept in g_scratch_ptr), never through the name.
+constexpr int kDiagBlocks = 1024;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[6 * kDiagBlocks];
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this GPU's scratch (looked up once per device, with that device
+// current).
+int scratch_of_current_device(double **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(SPHK_E_DEVICE, "diagnostics_static: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(SPHK_E_DEVICE, "diagnostics_static: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(SPHK_E_DEVICE, "diagnostics_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SPHK_OK;
+}
+
 // Scratch for the diagnostics partials: one per process and device ordinal.
 struct DiagScratch {
     double *part = nullptr;
@@ -425,7 +458,7 @@ int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *ou
     if (rc) return rc;
     if (!out) return set_error(SPHK_E_ARG, "diagnostics: null output");
     const int dev = sphk_device(ctx);
-    const int nb = 1024;
+    const int nb = kDiagBlocks;
     if (g_diag.device != dev || g_diag.cap < nb) {
         if (g_diag.part) (void)hipFree(g_diag.part);
         if (g_diag.out) (void)hipFree(g_diag.out);
@@ -445,6 +478,22 @@ int sphk_diagnostics(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *ou
         hipStreamSynchronize(st) != hipSuccess)
         return set_error(SPHK_E_DEVICE, "diagnostics: copy-back failed");
     return SPHK_OK;
+}
+
+int sphk_diagnostics_static(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, double *out_dev, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc) return rc;
+    if (!out_dev) return set_error(SPHK_E_ARG, "diagnostics_static: null output");
+    double *part = nullptr;
+    if ((rc = scratch_of_current_device(&part))) return rc;
+    hipStream_t st = stream_of(ctx, stream);
+    // the same kernels and grid as sphk_diagnostics: bitwise the same sums;
+    // every partial is written before it is read, nothing needs clearing
+    hipLaunchKernelGGL(diag_partial_kernel, dim3(kDiagBlocks), dim3(kBlock), 0, st, n, s, p, part);
+    if ((rc = launch_check("diagnostics_static"))) return rc;
+    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(kBlock), 0, st, kDiagBlocks, part, out_dev);
+    return launch_check("diagnostics_static");
 }
 
 int sphk_unbound_mass_dev(sphk_ctx *ctx, int n, sphk_state s, const int32_t *istar, int star, double k,
