@@ -62,12 +62,12 @@ def main() -> int:
     ap.add_argument("--eta", type=float, default=1.25, help="1.25 realises ~50 neighbours (ic.ETA_50)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="per functor")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the FDPS Dispatch/Retrieve (host buffer) leg")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --n-group for all kernels instead of tuning 8/16/32/64 per kernel")
     ap.add_argument("--groups", default=None,
-                    help="walk group sizes dens,hydro,grav[,density list capacity] (e.g. 32,32,32,256); "
-                         "implies --no-autotune")
+                    help="walk group sizes dens,hydro,grav[,density list capacity[,density members in "
+                         "registers 0/1]] (e.g. 32,32,32,256,1); implies --no-autotune")
     ap.add_argument("--no-kdk", action="store_true",
                     help="skip the full KDK step timing after the force step (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -118,7 +118,8 @@ def main() -> int:
     sysm = SPHSystem(p, ctx, eta=args.eta, eps=0.01, theta=0.5, n_group=n_group,
                      precision=MIXED)
     if args.groups and len(args.groups.split(",")) > 3:
-        ctx.set_params(list_cap=int(args.groups.split(",")[3]))
+        gv = args.groups.split(",")
+        ctx.set_params(list_cap=int(gv[3]), dens_members_in_registers=int(gv[4]) if len(gv) > 4 else 0)
     t0 = time.perf_counter()
     rounds = sysm.relax_h(3)
     sysm.eos_balsara()
@@ -227,7 +228,9 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sysm, entries, args.cpu_seconds, log)
+        cpu = cpu_baseline(sysm, entries, log)
+        cpu["cfg1_whole_step"] = cfg1_step(log)
+    dropin = None if args.no_dropin else dropin_leg(sysm, log)
 
     out = {
         "metric": METRIC, "value": total_entries / (ms_mixed * 1e-3), "unit": "interactions/s",
@@ -236,14 +239,17 @@ def main() -> int:
         "data": "synthetic (seeded Lane-Emden polytropes)",
         "config": {"workload": "off-axis parabolic collision n=3 (M=1,R=1) + n=1.5 (M=0.5,R=0.6), "
                                "b=0.25R at 3R, one density+hydro+gravity force step",
-                   "N": sysm.n, "eta": args.eta, "neighbours_target": 50, "theta": 0.5,
+                   "N": sysm.n, "eta": args.eta, "neighbours_target": 50,
+                   "neighbours_mean": round(float(sysm.np_.double().mean()), 2), "theta": 0.5,
                    "softening": 0.01, "n_group": dict(sysm.n_group), "n_leaf": 8,
                    "walks": {k: W[k].n_walk for k in W}},
         "autotune_ms": None if tune is None else {k: {str(g): round(t, 4) for g, t in v.items()}
                                                   for k, v in tune[1].items()},
         "dens_list_cap": sysm.ctx.params["list_cap"] or 256,
+        "dens_members_in_registers": int(sysm.ctx.params["dens_members_in_registers"]),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dropin_fdps_path": dropin,
         "entries": entries, "flop_per_step": flops,
         "kernel_ms": {k: round(v, 4) for k, v in kern_mixed.items()},
         "tflops_step": flops / (ms_mixed * 1e-3) / 1e12,
@@ -479,136 +485,196 @@ def profiled_traffic(kernel: str, n: int):
     return float(kv["traffic_bytes_per_launch"]), str(path.relative_to(ROOT))
 
 
-def cpu_baseline(sysm, entries, seconds, log):
-    """The oracle (scalar FP64 C, OpenMP over walks) on a bounded sample of the
-    same walks and lists; returns interactions/s scaled to the full step."""
-    import oracle
-    nthreads = min(16, os.cpu_count() or 1)
+def _host_inputs(sysm):
+    """The step's functor inputs and lists on the host (numpy)."""
     s = {k: v.cpu().numpy() for k, v in sysm.s.items()}
     ids = sysm.id.cpu().numpy()
-    rs = sysm.rs.cpu().numpy()
     hp = {k: v.cpu().numpy() for k, v in sysm.hprep.items()}
     eps2 = sysm.eps2.cpu().numpy()
     mom = {k: v.cpu().numpy() for k, v in sysm.mom.items()}
-    host = {}
-    for k, w in sysm.lists.items():
-        host[k] = [None if t is None else t.cpu().numpy() for t in (w.i_off, w.j_off, w.j_idx, w.s_off, w.s_idx)]
+    lists = {k: [None if t is None else t.cpu().numpy() for t in (w.i_off, w.j_off, w.j_idx, w.s_off, w.s_idx)]
+             for k, w in sysm.lists.items()}
     dsrc = dict(id=ids, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"], vz=s["vz"],
-                ksr=s["ksr"], rs=rs)
+                ksr=s["ksr"], rs=sysm.rs.cpu().numpy())
     hsrc = dict(id=ids, m=s["m"], x=s["x"], y=s["y"], z=s["z"], vx=s["vx"], vy=s["vy"], vz=s["vz"],
                 hinv=hp["hinv"], pres=s["pres"], presc=hp["presc"], bswt=hp["bswt_half"],
                 dens=s["dens"], vsnd=s["vsnd"], alph=s["alph"], alphu=s["alphu"], thrm=hp["thrm"],
                 eta=s["eta"])
     gsrc = dict(m=s["m"], x=s["x"], y=s["y"], z=s["z"], eps2=eps2)
+    return dsrc, hsrc, gsrc, mom, lists
 
-    def sample(kind, fn):
-        i_off, j_off, j_idx, s_off, s_idx = host[kind]
-        nw = len(i_off) - 1
-        ni = np.diff(i_off.astype(np.int64))
-        per = ni * np.diff(j_off.astype(np.int64))
-        if s_off is not None:
-            per = per + ni * np.diff(s_off.astype(np.int64))
-        done_e, done_t, frac = 0, 0.0, 0.002
-        # 8 evenly spaced blocks of consecutive walks, grown until the budget is used
-        while done_t < seconds and frac <= 1.0:
-            bl = max(1, int(nw * frac / 8))
-            for b in range(8):
-                w0 = min(nw - bl, b * nw // 8)
-                sl = slice(w0, w0 + bl + 1)
-                t = time.perf_counter()
-                fn(i_off[sl], j_off[sl], j_idx, None if s_off is None else s_off[sl], s_idx)
-                done_t += time.perf_counter() - t
-                done_e += int(per[w0:w0 + bl].sum())
-            frac *= 4
-        return done_e, done_t
 
-    res = {}
-    e, t = sample("dens", lambda io, jo, ji, so, si: oracle.density(io, jo, ji, dsrc, dsrc, eta=sysm.eta,
-                  hmax=sysm.ctx.params["hmax"], nthreads=nthreads))
-    res["density"] = (e, t)
-    e, t = sample("hydro", lambda io, jo, ji, so, si: oracle.hydro(io, jo, ji, hsrc, hsrc, nthreads=nthreads))
-    res["hydro"] = (e, t)
-    e, t = sample("grav", lambda io, jo, ji, so, si: oracle.gravity(io, jo, ji, gsrc, gsrc, s_off=so, s_idx=si,
-                  ps=mom, nthreads=nthreads))
-    res["gravity"] = (e, t)
-    step_s = sum(entries[k] / (res[k][0] / res[k][1]) for k in res)
-    total = sum(entries.values())
-    log("cpu baseline: " + ", ".join(f"{k} {res[k][0]:.3g} entries in {res[k][1]:.1f}s" for k in res))
+def _oracle_step(sysm, nthreads, inputs=None):
+    """The oracle's three functors over all walks of the system's lists;
+    returns (outputs per functor, seconds per functor)."""
+    import oracle
+    dsrc, hsrc, gsrc, mom, L = inputs or _host_inputs(sysm)
+    out, sec = {}, {}
+    t = time.perf_counter()
+    out["dens"] = oracle.density(*L["dens"][:3], dsrc, dsrc, eta=sysm.eta, hmax=sysm.ctx.params["hmax"],
+                                 nthreads=nthreads)
+    sec["density"] = time.perf_counter() - t
+    t = time.perf_counter()
+    out["hydro"] = oracle.hydro(*L["hydro"][:3], hsrc, hsrc, nthreads=nthreads)
+    sec["hydro"] = time.perf_counter() - t
+    i_off, j_off, j_idx, s_off, s_idx = L["grav"]
+    t = time.perf_counter()
+    out["grav"] = oracle.gravity(i_off, j_off, j_idx, gsrc, gsrc, s_off=s_off, s_idx=s_idx, ps=mom,
+                                 nthreads=nthreads)
+    sec["gravity"] = time.perf_counter() - t
+    return out, sec
 
-    # the oracle as the checker at full size: the functors' outputs (FP64 and
-    # mixed, all walks, same inputs) against the oracle's on sampled walks
+
+def _gpu_outputs(sysm, prec):
+    """The three functors on the device over the system's lists, into scratch
+    outputs (the state is not touched); host copies."""
     import torch
-    from research_codes_amd.device import FP64, MIXED
     W = sysm.lists
     gdev = dict(m=sysm.s["m"], x=sysm.s["x"], y=sysm.s["y"], z=sysm.s["z"], eps2=sysm.eps2)
-    gpu = {}
-    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
-        od = {k: torch.empty_like(sysm.s["m"]) for k in ("dens", "divv", "rotv", "grdh", "ksr")}
-        od["np"] = torch.empty_like(sysm.np_)
-        od["itr"] = torch.empty_like(sysm.itr)
-        sysm.ctx.density(W["dens"], sysm.dens_src(), sysm.dens_src(), od, prec)
-        oh = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx",
-                                                         "diffu")}
-        sysm.ctx.hydro(W["hydro"], sysm.hydro_src(), sysm.hydro_src(), oh, prec)
-        og = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "pot", "eta")}
-        sysm.ctx.gravity(W["grav"], gdev, gdev, og, ps=sysm.mom, precision=prec)
-        gpu[name] = {"dens": {k: v.cpu().numpy() for k, v in od.items()},
-                     "hydro": {k: v.cpu().numpy() for k, v in oh.items()},
-                     "grav": {k: v.cpu().numpy() for k, v in og.items()}}
-    ref = {"dens": {}, "hydro": {}, "grav": {}}
-    parts = []
+    od = {k: torch.empty_like(sysm.s["m"]) for k in ("dens", "divv", "rotv", "grdh", "ksr")}
+    od["np"] = torch.empty_like(sysm.np_)
+    od["itr"] = torch.empty_like(sysm.itr)
+    sysm.ctx.density(W["dens"], sysm.dens_src(), sysm.dens_src(), od, prec)
+    oh = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx",
+                                                     "diffu")}
+    sysm.ctx.hydro(W["hydro"], sysm.hydro_src(), sysm.hydro_src(), oh, prec)
+    og = {k: torch.empty_like(sysm.s["m"]) for k in ("ax", "ay", "az", "pot", "eta")}
+    sysm.ctx.gravity(W["grav"], gdev, gdev, og, ps=sysm.mom, precision=prec)
+    return {"dens": {k: v.cpu().numpy() for k, v in od.items()},
+            "hydro": {k: v.cpu().numpy() for k, v in oh.items()},
+            "grav": {k: v.cpu().numpy() for k, v in og.items()}}
+
+
+def _parity(gpu, ref):
+    errs, np_mis = {}, 0
     for kind in ("dens", "hydro", "grav"):
-        i_off, j_off, j_idx, s_off, s_idx = host[kind]
-        nw = len(i_off) - 1
-        for b in range(4):
-            sl = slice(b * nw // 4, b * nw // 4 + 33)
-            io, jo = i_off[sl], j_off[sl]
-            if kind == "dens":
-                o = oracle.density(io, jo, j_idx, dsrc, dsrc, eta=sysm.eta, hmax=sysm.ctx.params["hmax"],
-                                   nthreads=nthreads)
-            elif kind == "hydro":
-                o = oracle.hydro(io, jo, j_idx, hsrc, hsrc, nthreads=nthreads)
-            else:
-                o = oracle.gravity(io, jo, j_idx, gsrc, gsrc, s_off=s_off[sl], s_idx=s_idx, ps=mom,
-                                   nthreads=nthreads)
-            idx = np.arange(int(io[0]), int(io[-1]))
-            for k, v in o.items():
-                ref[kind].setdefault(k, []).append(v[idx])
-            if kind == "dens":
-                parts.append(idx)
-    ref = {kind: {k: np.concatenate(v) for k, v in d.items()} for kind, d in ref.items()}
+        for k, r in ref[kind].items():
+            if k == "itr":
+                continue
+            g = gpu[kind][k]
+            if k == "np":
+                np_mis += int((g != r).sum())
+                continue
+            # relative to the field's largest value
+            errs[f"{kind}.{k}"] = float(np.abs(g - r).max() / max(np.abs(r).max(), 1e-300))
+    return {"np_mismatch": np_mis, "max_rel_err": max(errs.values()), "worst_field": max(errs, key=errs.get),
+            "per_field": errs}
 
-    def sel(kind):
-        i_off = host[kind][0]
-        nw = len(i_off) - 1
-        return np.concatenate([np.arange(int(i_off[b * nw // 4]), int(i_off[b * nw // 4 + 32]))
-                               for b in range(4)])
 
-    parity = {"walks": 4 * 32, "particles": int(len(np.concatenate(parts)))}
-    for name in gpu:
-        errs, np_mis = {}, 0
-        for kind in ("dens", "hydro", "grav"):
-            ix = sel(kind)
-            for k, r in ref[kind].items():
-                if k == "itr":
-                    continue
-                g = gpu[name][kind][k][ix]
-                if k == "np":
-                    np_mis += int((g != r).sum())
-                    continue
-                # relative to the field's largest value over the sample
-                errs[f"{kind}.{k}"] = float(np.abs(g - r).max() / max(np.abs(r).max(), 1e-300))
-        parity[name] = {"np_mismatch": np_mis, "max_rel_err": max(errs.values()),
-                        "worst_field": max(errs, key=errs.get), "per_field": errs}
-    log(f"parity on sampled walks: fp64 max {parity['fp64']['max_rel_err']:.2e}, mixed max "
+def cpu_baseline(sysm, entries, log):
+    """The oracle (scalar FP64 C, OpenMP over walks on up to 16 host cores)
+    running the same force step -- the three functors over every walk of the
+    same lists and inputs -- timed whole, beside the CPU-runnable
+    configuration (N = 1e4, one step) timed whole; the GPU functors' outputs
+    (FP64 and mixed) are checked against the oracle's for every particle."""
+    from research_codes_amd.device import FP64, MIXED
+    nthreads = min(16, os.cpu_count() or 1)
+    ref, sec = _oracle_step(sysm, nthreads)
+    step_s = sum(sec.values())
+    total = sum(entries.values())
+    log("cpu baseline (oracle, whole step): " + ", ".join(f"{k} {v:.2f}s" for k, v in sec.items()))
+    parity = {"particles": sysm.n}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        parity[name] = _parity(_gpu_outputs(sysm, prec), ref)
+    log(f"parity on all particles: fp64 max {parity['fp64']['max_rel_err']:.2e}, mixed max "
         f"{parity['mixed']['max_rel_err']:.2e}, np mismatches {parity['fp64']['np_mismatch']}"
         f"/{parity['mixed']['np_mismatch']}")
     return {"value": total / step_s, "unit": "interactions/s", "cores": nthreads, "kind": "port",
-            "cpu": cpu_model(), "ms_per_step_est": round(step_s * 1e3, 1),
-            "sample": "oracle (scalar FP64 C, OpenMP over walks) on 8 evenly spaced blocks of "
-                      "consecutive walks per functor, same lists and inputs as the GPU step; "
-                      + ", ".join(f"{k}: {res[k][0]:.3g} entries / {res[k][1]:.1f}s" for k in res),
+            "cpu": cpu_model(), "ms_per_step": round(step_s * 1e3, 1),
+            "seconds_per_functor": {k: round(v, 3) for k, v in sec.items()},
+            "arithmetic": "scalar FP64 (the reference's shipped CPU path is its AVX2 branch, whose refined "
+                          "reciprocals differ from the scalar functor by < 1e-12: tests/test_oracle_avx2.py)",
+            "sample": "the whole force step: the oracle's density, hydro and gravity functors over all walks "
+                      "of the GPU step's lists and inputs, once, OpenMP over walks",
             "parity_vs_gpu": parity}
+
+
+def cfg1_step(log):
+    """The CPU-runnable configuration whole: one density + hydro + gravity
+    force step of an N = 1e4 n=1.5 star (eta = 1.25, theta = 0.5, constant
+    softening 0.01) on the GPU (mixed and FP64, HIP events around the step)
+    and by the oracle on the same lists and inputs (host cores)."""
+    import torch
+    from research_codes_amd import ic
+    from research_codes_amd.device import FP64, MIXED, Context
+    from research_codes_amd.step import SPHSystem
+    p = ic.single_polytrope(10_000, seed=1001)
+    with Context(0) as c:
+        s = SPHSystem(p, c, eta=ic.ETA_50, eps=0.01, theta=0.5, n_group=32, precision=MIXED)
+        s.relax_h(2)
+        s.converge_density(full_lists=True)
+        s.eos_balsara()
+        s.build_lists(("hydro", "grav"))
+        s.gravity()
+        s.hydro()
+        snap = s._sbuf.clone()
+        res = {"N": s.n, "entries": sum(int(w.entries + w.entries_spj) for w in s.lists.values())}
+        for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+            ms = []
+            for _ in range(6):
+                s._sbuf.copy_(snap)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                s.force(prec)
+                e1.record()
+                torch.cuda.synchronize()
+                ms.append(e0.elapsed_time(e1))
+            res[f"gpu_ms_{name}"] = round(float(np.median(ms[1:])), 4)
+        s._sbuf.copy_(snap)
+        nthreads = min(16, os.cpu_count() or 1)
+        _, sec = _oracle_step(s, nthreads)
+        res["oracle_ms"] = round(sum(sec.values()) * 1e3, 2)
+        res["oracle_cores"] = nthreads
+        res["mean_np"] = round(float(s.np_.double().mean()), 2)
+    log(f"N=1e4 step: GPU {res['gpu_ms_mixed']} ms mixed, {res['gpu_ms_fp64']} ms FP64; "
+        f"oracle {res['oracle_ms']} ms")
+    return res
+
+
+def dropin_leg(sysm, log, reps=1):
+    """The drop-in path a reference user gets: the reference's
+    DispatchKernel_* / RetrieveKernel_* (libsphk.so's FDPS adapter, host AoS
+    buffers, PCIe both ways) driven by tests/fdps_driver the way FDPS drives
+    them (<= 200 walks per call), over the same walks and lists as the
+    device-resident step.  Returns the driver's timings (ms per functor pass,
+    AoS construction excluded) and bytes moved."""
+    import subprocess
+    import tempfile
+    drv = ROOT / "tests" / "fdps_driver"
+    if not drv.exists():
+        return None
+    dsrc, hsrc, gsrc, mom, L = _host_inputs(sysm)
+    arrays = dict(params=np.array([sysm.eta, 1.936492, sysm.ctx.params["hmax"]]), id=dsrc["id"].astype(np.int32),
+                  m=dsrc["m"], x=dsrc["x"], y=dsrc["y"], z=dsrc["z"], vx=dsrc["vx"], vy=dsrc["vy"], vz=dsrc["vz"],
+                  ksr=dsrc["ksr"], rs=dsrc["rs"], hinv=hsrc["hinv"], pres=hsrc["pres"], presc=hsrc["presc"],
+                  bswt=hsrc["bswt"], dens=hsrc["dens"], vsnd=hsrc["vsnd"], alph=hsrc["alph"],
+                  alphu=hsrc["alphu"], thrm=hsrc["thrm"], eta=hsrc["eta"], eps2=gsrc["eps2"],
+                  spj_m=mom["m"], spj_x=mom["x"], spj_y=mom["y"], spj_z=mom["z"], spj_eps2=mom["eps2"])
+    for kind, name in (("dens", "dens"), ("hydro", "hydro"), ("grav", "grav")):
+        io, jo, ji, so, si = L[kind]
+        arrays[f"{name}_i_off"], arrays[f"{name}_j_off"], arrays[f"{name}_j_idx"] = io, jo, ji
+        if kind == "grav":
+            arrays["grav_s_off"], arrays["grav_s_idx"] = so, si
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        with open(Path(d) / "manifest.txt", "w") as mf:
+            for k, v in arrays.items():
+                v = np.ascontiguousarray(v, np.int32 if (k == "id" or k.endswith(("_off", "_idx"))) else None)
+                v.tofile(Path(d) / f"{k}.bin")
+                mf.write(f"{k} {v.nbytes}\n")
+        env = dict(os.environ, SPHK_PRECISION="mixed")
+        log("drop-in path: fdps_driver over the same walks")
+        r = subprocess.run([str(drv), d, str(reps)], env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log(f"fdps_driver failed: {r.stdout[-500:]} {r.stderr[-500:]}")
+        return None
+    res = json.loads(next(line for line in r.stdout.splitlines() if line.startswith("{")))
+    step_ms = res["density_ms"] + res["hydro_ms"] + res["gravity_ms"]
+    moved = sum(res["h2d_bytes"]) + sum(res["d2h_bytes"])
+    res.update(ms_per_step=round(step_ms, 2), pcie_bytes_per_step=moved,
+               pcie_inclusive_gb_per_s=round(moved / (step_ms * 1e-3) / 1e9, 2),
+               precision="mixed", note="host AoS in, host AoS out, <= 200 walks per Dispatch, tag 0")
+    log(f"drop-in path: {step_ms:.1f} ms per step, {moved / 1e9:.2f} GB over PCIe")
+    return res
 
 
 if __name__ == "__main__":

@@ -530,6 +530,11 @@ __global__ __launch_bounds__(kWave) void density_vjp_kernel(int n_walk, const in
 
         // -- reverse through the outputs (per particle)
         const double cD = cot.dens[i], cK = cot.ksr[i], cV = cot.divv[i], cR = cot.rotv[i], cG = cot.grdh[i];
+        // a particle whose outputs carry no cotangent contributes nothing;
+        // it is skipped outright, so that outputs it does not own (e.g. an
+        // empty sum over a list built for other particles: 0 * inf) cannot
+        // turn into NaN cotangents of its neighbours
+        const bool live = cD != 0. || cK != 0. || cV != 0. || cR != 0. || cG != 0.;
         const double Omb = cG + (cV * dvv + cR * rotm) / rho2;
         double rho2b = cD - (cV * dvv + cR * rotm) * Om / (rho2 * rho2);
         const double A = Om / rho2;
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(kWave) void density_vjp_kernel(int n_walk, const in
             const double r2 = dx * dx + dy * dy + dz * dz;
             const double rr = __builtin_sqrt(r2);
             const double q = rr * h3;
-            if (!(q < 1.)) return;
+            if (!live || !(q < 1.)) return;
             const double t = 1. - q;
             const double t2 = t * t, t3 = t2 * t;
             const double dw = k5 * r.m * h3_5 * t3;
@@ -615,6 +620,7 @@ __global__ __launch_bounds__(kWave) void density_vjp_kernel(int n_walk, const in
             wk[k] = rhob[k] * k5 * (h2 * h2 * hk[k]);   // rho_bar h^5 (-5 C1)
         }
         sweep([&](const DRec &r, int k) {
+            if (!live) return;
             const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
             const double rr = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
             double c = 0.;
@@ -637,7 +643,7 @@ __global__ __launch_bounds__(kWave) void density_vjp_kernel(int n_walk, const in
 
         // -- i-side totals
         slice_total<6>(gi, red, lane, il, ni, S);
-        if (lead) {
+        if (lead && live) {
             atomicAdd(&grad.x[i], gi[0]);
             atomicAdd(&grad.y[i], gi[1]);
             atomicAdd(&grad.z[i], gi[2]);

@@ -3,7 +3,8 @@
 // ABI of libsphk.so.
 //
 // Dispatch copies each walk's AoS EPI/EPJ arrays (owned by FDPS, valid only
-// during the call) into per-tag structure-of-arrays host buffers and calls
+// during the call) into per-tag structure-of-arrays host buffers, walks in
+// parallel (OpenMP), and calls
 // sphk_*_dispatch, which stages them into pinned memory, copies them to the
 // device in one transfer and launches asynchronously.  Retrieve waits for
 // the result and scatters it back into FDPS's per-walk Force arrays in walk
@@ -75,10 +76,11 @@ TagBuf &tagbuf(int tag)
     return g_tag[tag];
 }
 
-size_t total(int n_walk, const int *cnt)
+// Start of each walk's rows in the packed arrays (exclusive prefix sum).
+std::vector<size_t> starts(int n_walk, const int *cnt)
 {
-    size_t s = 0;
-    for (int w = 0; w < n_walk; w++) s += (size_t)cnt[w];
+    std::vector<size_t> s(n_walk + 1, 0);
+    for (int w = 0; w < n_walk; w++) s[w + 1] = s[w] + (size_t)cnt[w];
     return s;
 }
 
@@ -120,11 +122,15 @@ int DispatchKernel_Dens(const int tag, const int n_walk, const DensityEPI **epi,
         int rc = sphk_set_params(g_ctx, &prm);
         if (rc) die("DispatchKernel_Dens (hmax)", rc);
     }
-    const size_t ni = total(n_walk, nip), nj = total(n_walk, njp);
+    const std::vector<size_t> si = starts(n_walk, nip), sj = starts(n_walk, njp);
+    const size_t ni = si[n_walk], nj = sj[n_walk];
     b.di.resize(9, ni);
     b.dj.resize(7, nj);
-    size_t a = 0, c = 0;
+    // walks are packed in parallel (the reference packs with OpenMP,
+    // hdr_densityGPU.cu:537)
+#pragma omp parallel for schedule(dynamic, 4)
     for (int w = 0; w < n_walk; w++) {
+        size_t a = si[w], c = sj[w];
         for (int i = 0; i < nip[w]; i++, a++) {
             const DensityEPI &e = epi[w][i];
             b.di.id[a] = e.id;
@@ -163,7 +169,8 @@ int RetrieveKernel_Dens(const int tag, const int n_walk, const int *ni, Density 
 {
     need_ctx();
     TagBuf &b = tagbuf(tag);
-    const size_t n = total(n_walk, ni);
+    const std::vector<size_t> si = starts(n_walk, ni);
+    const size_t n = si[n_walk];
     for (int k = 0; k < 5; k++) b.o[k].resize(n);
     b.onp.resize(n);
     b.oitr.resize(n);
@@ -171,8 +178,9 @@ int RetrieveKernel_Dens(const int tag, const int n_walk, const int *ni, Density 
                     b.onp.data(),  b.oitr.data()};
     int rc = sphk_density_retrieve(g_ctx, tag, n_walk, ni, o);
     if (rc) die("RetrieveKernel_Dens", rc);
-    size_t a = 0;
-    for (int w = 0; w < n_walk; w++)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int w = 0; w < n_walk; w++) {
+        size_t a = si[w];
         for (int i = 0; i < ni[w]; i++, a++) {
             Density &d = density[w][i];
             d.dens = b.o[0][a];
@@ -183,6 +191,7 @@ int RetrieveKernel_Dens(const int tag, const int n_walk, const int *ni, Density 
             d.np = b.onp[a];
             d.itr = b.oitr[a] != 0;
         }
+    }
     return 0;
 }
 
@@ -211,11 +220,12 @@ int DispatchKernel_HydroForce(const int tag, const int n_walk, const HydroEPI **
 {
     need_ctx();
     TagBuf &b = tagbuf(tag);
-    const size_t ni = total(n_walk, nip), nj = total(n_walk, njp);
-    b.hi.resize(17, ni);
-    b.hj.resize(17, nj);
-    size_t a = 0, c = 0;
+    const std::vector<size_t> si = starts(n_walk, nip), sj = starts(n_walk, njp);
+    b.hi.resize(17, si[n_walk]);
+    b.hj.resize(17, sj[n_walk]);
+#pragma omp parallel for schedule(dynamic, 4)
     for (int w = 0; w < n_walk; w++) {
+        size_t a = si[w], c = sj[w];
         for (int i = 0; i < nip[w]; i++, a++) put_hydro(b.hi, a, epi[w][i], 0.0);
         for (int j = 0; j < njp[w]; j++, c++) put_hydro(b.hj, c, epj[w][j], epj[w][j].mass);
     }
@@ -229,14 +239,16 @@ int RetrieveKernel_HydroForce(const int tag, const int n_walk, const int *ni, Hy
 {
     need_ctx();
     TagBuf &b = tagbuf(tag);
-    const size_t n = total(n_walk, ni);
+    const std::vector<size_t> si = starts(n_walk, ni);
+    const size_t n = si[n_walk];
     for (int k = 0; k < 9; k++) b.o[k].resize(n);
     sphk_hydro_out o{b.o[0].data(), b.o[1].data(), b.o[2].data(), b.o[3].data(), b.o[4].data(),
                      b.o[5].data(), b.o[6].data(), b.o[7].data(), b.o[8].data()};
     int rc = sphk_hydro_retrieve(g_ctx, tag, n_walk, ni, o);
     if (rc) die("RetrieveKernel_HydroForce", rc);
-    size_t a = 0;
-    for (int w = 0; w < n_walk; w++)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int w = 0; w < n_walk; w++) {
+        size_t a = si[w];
         for (int i = 0; i < ni[w]; i++, a++) {
             Hydro &h = hydro[w][i];
             h.acch = {b.o[0][a], b.o[1][a], b.o[2][a]};
@@ -245,6 +257,7 @@ int RetrieveKernel_HydroForce(const int tag, const int n_walk, const int *ni, Hy
             h.vsmx = b.o[7][a];
             h.diffu = b.o[8][a];
         }
+    }
     return 0;
 }
 
@@ -254,13 +267,14 @@ int DispatchKernel_Gravity(const int tag, const int n_walk, const GravityEPI **e
 {
     need_ctx();
     TagBuf &b = tagbuf(tag);
-    const size_t ni = total(n_walk, nip), nj = total(n_walk, njp);
-    const size_t ns = nsp ? total(n_walk, nsp) : 0;
-    b.gi.resize(5, ni);
-    b.gj.resize(5, nj);
-    b.gs.resize(5, ns);
-    size_t a = 0, c = 0, d = 0;
+    const std::vector<size_t> si = starts(n_walk, nip), sj = starts(n_walk, njp);
+    const std::vector<size_t> ss = nsp ? starts(n_walk, nsp) : std::vector<size_t>(n_walk + 1, 0);
+    b.gi.resize(5, si[n_walk]);
+    b.gj.resize(5, sj[n_walk]);
+    b.gs.resize(5, ss[n_walk]);
+#pragma omp parallel for schedule(dynamic, 4)
     for (int w = 0; w < n_walk; w++) {
+        size_t a = si[w], c = sj[w], d = ss[w];
         for (int i = 0; i < nip[w]; i++, a++) {
             const GravityEPI &e = epi[w][i];
             b.gi.f[0][a] = 0.0;
@@ -299,18 +313,21 @@ int RetrieveKernel_Gravity(const int tag, const int n_walk, const int *ni, Gravi
 {
     need_ctx();
     TagBuf &b = tagbuf(tag);
-    const size_t n = total(n_walk, ni);
+    const std::vector<size_t> si = starts(n_walk, ni);
+    const size_t n = si[n_walk];
     for (int k = 0; k < 5; k++) b.o[k].resize(n);
     sphk_grav_out o{b.o[0].data(), b.o[1].data(), b.o[2].data(), b.o[3].data(), b.o[4].data()};
     int rc = sphk_gravity_retrieve(g_ctx, tag, n_walk, ni, o);
     if (rc) die("RetrieveKernel_Gravity", rc);
-    size_t a = 0;
-    for (int w = 0; w < n_walk; w++)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int w = 0; w < n_walk; w++) {
+        size_t a = si[w];
         for (int i = 0; i < ni[w]; i++, a++) {
             Gravity &g = gravity[w][i];
             g.acc = {b.o[0][a], b.o[1][a], b.o[2][a]};
             g.pot = b.o[3][a];
             g.eta = b.o[4][a];
         }
+    }
     return 0;
 }

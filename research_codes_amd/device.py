@@ -82,12 +82,26 @@ class Walks:
                 raise TypeError("device walk tables must be int32 CUDA tensors")
         self.n_walk = i_off.numel() - 1
         self.i_off, self.j_off, self.j_idx, self.s_off, self.s_idx = i_off, j_off, j_idx, s_off, s_idx
-        ni = (i_off[1:] - i_off[:-1]).long()
-        self.n_i = int(i_off[-1])
-        self.n_j = int(j_off[-1])
-        self.n_s = 0 if s_off is None else int(s_off[-1])
-        self.entries = int((ni * (j_off[1:] - j_off[:-1]).long()).sum())
-        self.entries_spj = 0 if s_off is None else int((ni * (s_off[1:] - s_off[:-1]).long()).sum())
+        self._sizes = None   # read from the device on first use (a host synchronisation)
+
+    def _read_sizes(self):
+        if self._sizes is None:
+            i_off, j_off, s_off = self.i_off, self.j_off, self.s_off
+            ni = (i_off[1:] - i_off[:-1]).long()
+            v = [i_off[-1].long(), j_off[-1].long(), (ni * (j_off[1:] - j_off[:-1]).long()).sum()]
+            if s_off is not None:
+                v += [s_off[-1].long(), (ni * (s_off[1:] - s_off[:-1]).long()).sum()]
+            h = torch.stack(v).cpu().tolist()
+            self._sizes = dict(n_i=h[0], n_j=h[1], entries=h[2], n_s=h[3] if len(h) > 3 else 0,
+                               entries_spj=h[4] if len(h) > 3 else 0)
+        return self._sizes
+
+    def __getattr__(self, name):
+        # sizes of device-built tables (interaction-list entries = sum over
+        # walks of n_i * n_j), computed lazily: list builds need not wait
+        if name in ("n_i", "n_j", "n_s", "entries", "entries_spj") and "_sizes" in self.__dict__:
+            return self._read_sizes()[name]
+        raise AttributeError(name)
 
 
 class Context:
@@ -121,7 +135,8 @@ class Context:
         except Exception:
             pass
 
-    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None, list_cap=None):
+    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None, list_cap=None,
+                   dens_members_in_registers=None):
         p = L.Params()
         if getattr(self, "_params", None) is not None:
             p = self._params
@@ -135,13 +150,16 @@ class Context:
             p.uniform_softening = int(bool(uniform_softening))
         if list_cap is not None:
             p.list_cap = int(list_cap)
+        if dens_members_in_registers is not None:
+            p.dens_members_in_registers = int(bool(dens_members_in_registers))
         check(lib().sphk_set_params(self._h, C.byref(p)), "sphk_set_params")
         self._params = p
 
     @property
     def params(self):
         return dict(eta=self._params.eta, ksrh=self._params.ksrh, hmax=self._params.hmax,
-                    uniform_softening=bool(self._params.uniform_softening), list_cap=self._params.list_cap)
+                    uniform_softening=bool(self._params.uniform_softening), list_cap=self._params.list_cap,
+                    dens_members_in_registers=bool(self._params.dens_members_in_registers))
 
     @staticmethod
     def _stream():

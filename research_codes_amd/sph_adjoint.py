@@ -392,8 +392,15 @@ def force_t(P: dict, m, rec: dict, sysm: SPHSystem) -> dict:
     ph = sysm.phys
     ksrh = ph.ksrh
     # density: the device functor and its hand-written VJP over the recorded
-    # gather lists, at the step's smoothing-length cap
-    dens, H, divv, rotv, omg = density_dev(rec["dens"], sysm.ctx, rec["hmax"], m, rec["id"], P)
+    # gather lists, at the step's smoothing-length cap; each particle takes
+    # the results of the retry round in which it converged (over that
+    # round's lists, as calcDensityKernel copies them)
+    rounds = rec.get("dens_rounds") or [(rec["dens"], None)]
+    res = None
+    for walks, took in rounds:
+        r = density_dev(walks, sysm.ctx, rec["hmax"], m, rec["id"], P)
+        res = r if res is None else tuple(torch.where(took, a, b) for a, b in zip(r, res))
+    dens, H, divv, rotv, omg = res
     # EOS and Balsara switch
     u = P["uene"]
     pres = ph.ninv * dens * u

@@ -277,17 +277,19 @@ class SPHSystem:
             for g in candidates:
                 self.n_group[kind] = g
                 self.build_lists((kind,))
-                if kind == "dens":   # the density kernel's resident list capacity too
+                if kind == "dens":   # the density kernel's list capacity and member layout too
                     for cap in (192, 256):
-                        self.ctx.set_params(list_cap=cap)
-                        table[kind][f"{g}/{cap}"] = timed(run[kind])
+                        for rc in (0, 1):
+                            self.ctx.set_params(list_cap=cap, dens_members_in_registers=rc)
+                            table[kind][f"{g}/{cap}/{rc}"] = timed(run[kind])
                 else:
                     table[kind][str(g)] = timed(run[kind])
             key = min(table[kind], key=table[kind].get)
             if kind == "dens":
-                g, cap = (int(v) for v in key.split("/"))
-                self.ctx.set_params(list_cap=cap)
+                g, cap, rc = (int(v) for v in key.split("/"))
+                self.ctx.set_params(list_cap=cap, dens_members_in_registers=rc)
                 best["dens_list_cap"] = cap
+                best["dens_members_in_registers"] = rc
             else:
                 g = int(key)
             best[kind] = g
@@ -321,7 +323,7 @@ class SPHSystem:
         return out
 
     def converge_density(self, expand: float = 1.1, max_rounds: int = 100, jump: bool = False,
-                         full_lists: bool = False):
+                         full_lists: bool = False, rounds_out: list | None = None):
         """calcDensityKernel (hdr_density.hpp:369-415) with device list rebuilds.
 
         rs = 1.1 ksr; each round rebuilds the gather lists and runs the density
@@ -336,9 +338,12 @@ class SPHSystem:
         full_lists=True leaves each particle's rs at the value it converged
         with and, when more than one round ran, rebuilds the gather lists
         with them, so that the density functor can be re-run on the converged
-        state (the reference leaves rs = 0).  Returns the number of rounds;
-        raises if particles still retry after max_rounds (the reference loops
-        without a bound).
+        state (the reference leaves rs = 0).  rounds_out (a list) receives
+        each round's gather lists and the mask of the particles that took that
+        round's results: a particle whose intermediate H outgrew rs has
+        results that depend on its round's list.  Returns the number of
+        rounds; raises if particles still retry after max_rounds (the
+        reference loops without a bound).
         """
         n = self.n
         s = self.s
@@ -354,6 +359,8 @@ class SPHSystem:
             active = self.rs != 0.0
             newly = active & (tmp["itr"] == 0)
             retry = active & ~newly
+            if rounds_out is not None:
+                rounds_out.append((self.lists["dens"], newly.clone()))
             for k in ("dens", "divv", "rotv", "grdh", "ksr"):
                 s[k].copy_(torch.where(newly, tmp[k], s[k]))
             self.np_.copy_(torch.where(newly, tmp["np"], self.np_))
@@ -489,14 +496,15 @@ class SPHSystem:
     def initialize(self, record: dict | None = None):
         """startSimulation's first force evaluation (hdr_hgas.hpp:1335-1378)."""
         hmax = self.hmax
-        self.converge_density(full_lists=record is not None)
+        rounds = [] if record is not None else None
+        self.converge_density(rounds_out=rounds)
         self.eos_balsara()
         self.build_lists(("hydro", "grav"))
         self.gravity()
         self.hydro()
         if record is not None:
-            record.update(perm=None, hmax=hmax, dens=self.lists["dens"], hydro=self.lists["hydro"],
-                          id=self.id.clone(), **self._grav_record())
+            record.update(perm=None, hmax=hmax, dens=self.lists["dens"], dens_rounds=rounds,
+                          hydro=self.lists["hydro"], id=self.id.clone(), **self._grav_record())
         self.update_hmax()
 
     def _grav_record(self) -> dict:
@@ -524,13 +532,14 @@ class SPHSystem:
         check(lib().sphk_predict_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),
               "sphk_predict_dev")
         self.resort()
-        self.converge_density(full_lists=record is not None)
+        rounds = [] if record is not None else None
+        self.converge_density(rounds_out=rounds)
         self.eos_balsara()
         self.build_lists(("hydro", "grav"))
         self.gravity()
         self.hydro()
         if record is not None:
-            record.update(perm=self.last_perm, hmax=hmax, dens=self.lists["dens"],
+            record.update(perm=self.last_perm, hmax=hmax, dens=self.lists["dens"], dens_rounds=rounds,
                           hydro=self.lists["hydro"], id=self.id.clone(), **self._grav_record())
         self.update_hmax()
         check(lib().sphk_correct_dev(self.ctx.handle, self.n, self._ss, dt, self._st()),

@@ -5,8 +5,13 @@
 // per-walk Force arrays (hdr_density.hpp:387-394, main.cpp:337-344,
 // hdr_gravity.hpp:390-397).
 //
-// usage: fdps_driver <dir>   -- reads <dir>/manifest.txt and <name>.bin raw
-// arrays written by tests/test_gpu_fdps_boundary.py, writes out_<name>.bin.
+// usage: fdps_driver <dir> [reps]  -- reads <dir>/manifest.txt and <name>.bin
+// raw arrays (written by tests/test_gpu_fdps_boundary.py or bench.py), writes
+// out_<name>.bin.  With reps > 0 every functor's multiwalk pass runs once
+// untimed and then reps times with its Dispatch + Retrieve calls timed (the
+// per-walk AoS copies, FDPS's own work, are outside the timing), and one
+// JSON line reports ms per pass and the bytes each pass moves across PCIe.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -77,7 +82,12 @@ int main(int argc, char **argv)
         return 2;
     }
     g_dir = argv[1];
+    const int reps = argc > 2 ? std::atoi(argv[2]) : 0;
     load(g_dir);
+    using clk = std::chrono::steady_clock;
+    double t_ms[3] = {0, 0, 0};
+    double h2d[3] = {0, 0, 0}, d2h[3] = {0, 0, 0};   // bytes per pass
+    auto since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
     const double *par = arr<double>("params");  // eta, ksrh, hmax
     size_t n;
     const int32_t *id = arr<int32_t>("id", &n);
@@ -111,6 +121,7 @@ int main(int argc, char **argv)
     std::vector<double> o_dens(n), o_divv(n), o_rotv(n), o_grdh(n), o_ksr(n);
     std::vector<long long> o_np(n);
     std::vector<unsigned char> o_itr(n);
+    for (int rep = 0; rep <= reps; rep++)
     for (size_t w0 = 0; w0 < nw_d; w0 += kWalkLimit) {
         const int nb = (int)std::min<size_t>(kWalkLimit, nw_d - w0);
         std::vector<std::vector<DensityEPI>> epi(nb);
@@ -135,8 +146,16 @@ int main(int argc, char **argv)
             nip[b] = (int)epi[b].size();
             njp[b] = (int)epj[b].size();
         }
+        const auto t0 = clk::now();
         DispatchKernel_Dens(0, nb, pe.data(), nip.data(), pj.data(), njp.data());
         RetrieveKernel_Dens(0, nb, nip.data(), pf.data());
+        if (rep > 0) t_ms[0] += since(t0);
+        if (rep == 1)
+            for (int b = 0; b < nb; b++) {
+                // SoA staged by sphk_density_dispatch: id + 9 reals per i, id + 7 per j; 5 reals, np, itr back
+                h2d[0] += nip[b] * 76.0 + njp[b] * 60.0 + 8.0;
+                d2h[0] += nip[b] * 49.0;
+            }
         for (int b = 0; b < nb; b++)
             for (int i = 0; i < nip[b]; i++) {
                 const size_t g = d_ioff[w0 + b] + i;
@@ -151,6 +170,7 @@ int main(int argc, char **argv)
     // ---- hydro (calcForceAllAndWriteBackMultiWalk)
     std::vector<double> h[9];
     for (auto &v : h) v.resize(n);
+    for (int rep = 0; rep <= reps; rep++)
     for (size_t w0 = 0; w0 < nw_h; w0 += kWalkLimit) {
         const int nb = (int)std::min<size_t>(kWalkLimit, nw_h - w0);
         std::vector<std::vector<HydroEPI>> epi(nb);
@@ -179,8 +199,15 @@ int main(int argc, char **argv)
             nip[b] = (int)epi[b].size();
             njp[b] = (int)epj[b].size();
         }
+        const auto t0 = clk::now();
         DispatchKernel_HydroForce(0, nb, pe.data(), nip.data(), pj.data(), njp.data());
         RetrieveKernel_HydroForce(0, nb, nip.data(), pf.data());
+        if (rep > 0) t_ms[1] += since(t0);
+        if (rep == 1)
+            for (int b = 0; b < nb; b++) {
+                h2d[1] += (nip[b] + njp[b]) * 140.0 + 8.0;   // id + 17 reals per record
+                d2h[1] += nip[b] * 72.0;
+            }
         for (int b = 0; b < nb; b++)
             for (int i = 0; i < nip[b]; i++) {
                 const size_t g = h_ioff[w0 + b] + i;
@@ -196,6 +223,7 @@ int main(int argc, char **argv)
     // ---- gravity (calcForceMakingTreeMultiWalk)
     std::vector<double> gr[5];
     for (auto &v : gr) v.resize(n);
+    for (int rep = 0; rep <= reps; rep++)
     for (size_t w0 = 0; w0 < nw_g; w0 += kWalkLimit) {
         const int nb = (int)std::min<size_t>(kWalkLimit, nw_g - w0);
         std::vector<std::vector<GravityEPI>> epi(nb);
@@ -228,8 +256,15 @@ int main(int argc, char **argv)
             njp[b] = (int)epj[b].size();
             nsp[b] = (int)spj[b].size();
         }
+        const auto t0 = clk::now();
         DispatchKernel_Gravity(0, nb, pe.data(), nip.data(), pj.data(), njp.data(), ps.data(), nsp.data());
         RetrieveKernel_Gravity(0, nb, nip.data(), pf.data());
+        if (rep > 0) t_ms[2] += since(t0);
+        if (rep == 1)
+            for (int b = 0; b < nb; b++) {
+                h2d[2] += (nip[b] + njp[b] + nsp[b]) * 40.0 + 12.0;   // 5 reals per record
+                d2h[2] += nip[b] * 40.0;
+            }
         for (int b = 0; b < nb; b++)
             for (int i = 0; i < nip[b]; i++) {
                 const size_t g = g_ioff[w0 + b] + i;
@@ -241,6 +276,11 @@ int main(int argc, char **argv)
     for (int k = 0; k < 5; k++) out(gn[k], gr[k]);
     reset_gpu(0);
     (void)nspj;
+    if (reps > 0)
+        std::printf("{\"density_ms\": %.3f, \"hydro_ms\": %.3f, \"gravity_ms\": %.3f, \"h2d_bytes\": [%.0f, %.0f, %.0f], "
+                    "\"d2h_bytes\": [%.0f, %.0f, %.0f], \"walk_limit\": %d, \"reps\": %d}\n",
+                    t_ms[0] / reps, t_ms[1] / reps, t_ms[2] / reps, h2d[0], h2d[1], h2d[2], d2h[0], d2h[1], d2h[2],
+                    kWalkLimit, reps);
     std::printf("fdps_driver: %zu particles, %zu/%zu/%zu walks ok\n", n, nw_d, nw_h, nw_g);
     return 0;
 }

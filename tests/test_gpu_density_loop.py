@@ -6,9 +6,9 @@ device H_max against the oracle's calcKernelSupportRadiusMaximum
 
 A third of the particles start at 0.55 of their H so that the loop runs
 several rounds, converged particles drop out with rs = 0 and the group lists
-shrink.  Bars: round count and np identical; dens, ksr, grdh identical to
-1e-12 (the h-iteration is FP64 in both precisions); divv and rotv within
-1e-12 (FP64) and 1e-4 (mixed) of their scale.
+shrink.  Bars: round count and np identical; dens and ksr to 1e-12 (the
+h-iteration is FP64 in both precisions); grdh, divv and rotv within 1e-12
+(FP64) and 1e-4 (mixed) of their scale.
 """
 import numpy as np
 import pytest
@@ -62,10 +62,11 @@ def test_retry_loop_matches_oracle_restatement(prec):
     print(f"[density loop] rounds {rounds} oracle {o_rounds}", flush=True)
     assert rounds == o_rounds > 2
     assert np.array_equal(np_got, o_np)
-    for k in ("dens", "ksr", "grdh"):
+    for k in ("dens", "ksr"):
         assert scaled_err(got[k], o[k]) < 1e-12, k
+    # the grad-h pass (Omega^-1, div v, curl v) runs in the pair precision
     tol = 1e-12 if prec == FP64 else 1e-4
-    for k in ("divv", "rotv"):
+    for k in ("grdh", "divv", "rotv"):
         assert scaled_err(got[k], o[k], 1e-1) < tol, k
 
 

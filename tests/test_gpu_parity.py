@@ -37,11 +37,12 @@ def case(request):
     return polytrope_case(N, seed=1000 + N, n_group=g)
 
 
+@pytest.mark.parametrize("rc", [0, 1], ids=["idx", "regs"])
 @pytest.mark.parametrize("cap", [256, 192], ids=["cap256", "cap192"])
 @pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
-def test_density(ctx, case, prec, cap):
+def test_density(ctx, case, prec, cap, rc):
     p = case["p"]
-    ctx.set_params(eta=case["eta"], hmax=case["hmax"], list_cap=cap)
+    ctx.set_params(eta=case["eta"], hmax=case["hmax"], list_cap=cap, dens_members_in_registers=rc)
     w = D.Walks(case["i_off"], *case["dens_lists"])
     src = _dev(p, D.L.DENS_SRC)
     out = D.empty_dens_out(len(p["m"]))
@@ -49,7 +50,7 @@ def test_density(ctx, case, prec, cap):
         ctx.density(w, src, src, out, prec)
         torch.cuda.synchronize()
     finally:
-        ctx.set_params(list_cap=0)
+        ctx.set_params(list_cap=0, dens_members_in_registers=0)
     ref = case["dens_out"]
     got = {k: v.cpu().numpy() for k, v in out.items()}
     assert np.array_equal(got["np"], ref["np"]), "neighbour counts differ"
@@ -153,12 +154,22 @@ def _density_vs_oracle(ctx, p, i_off, j_off, j_idx, eta, hmax, prec, label):
     return n_itr
 
 
+@pytest.mark.parametrize("rc", [0, 1], ids=["idx", "regs"])
 @pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
-def test_density_compaction_paths(ctx, prec):
-    """The compacted density sweeps against the oracle where they switch
-    paths: a search radius equal to H (many H_k exceed it: full-sweep
-    fallback, itr set), a wide radius (compacted lists overflow one chunk),
-    and a walk of 125 particles whose list fits in LDS (i-chunks > 64)."""
+def test_density_compaction_paths(ctx, prec, rc):
+    """The compacted density sweeps (both member layouts) against the oracle
+    where they switch paths: a search radius equal to H (many H_k exceed it:
+    full-sweep fallback, itr set), a wide radius (compacted lists overflow
+    one chunk), and a walk of 125 particles whose list fits in LDS (i-chunks
+    > 64)."""
+    ctx.set_params(dens_members_in_registers=rc)
+    try:
+        _compaction_paths(ctx, prec)
+    finally:
+        ctx.set_params(dens_members_in_registers=0)
+
+
+def _compaction_paths(ctx, prec):
     from research_codes_amd.tree import HostTree, sort_particles
     p = ic.polytrope(1.5, 3000, 1.0, 1.0, seed=42, eta=1.6)
     p, _ = sort_particles(p)

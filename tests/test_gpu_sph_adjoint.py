@@ -75,13 +75,14 @@ def test_collision_gradient_matches_finite_differences(k):
 def test_long_rollout_gradient_matches_finite_differences():
     """200 steps with the power-of-two timestep (held fixed by the gradient
     and replayed by the differences), stars in contact at t=0, direct-sum
-    gravity: the contact shock makes the loss O(0.1)."""
+    gravity.  The loss's slope is k = 2: by t ~ 0.24 no mass is unbound, and
+    the k = 20 loss is the 1e-11 sigmoid tail; at k = 2 it is O(0.1)."""
     n_each, steps = 3000, 200
-    loss, g, dts = head_on_gradient(n_each=n_each, steps=steps, dt=None, k=20.0, theta=0.0,
+    loss, g, dts = head_on_gradient(n_each=n_each, steps=steps, dt=None, k=2.0, theta=0.0,
                                     separation=2.0)
 
     def f(dx0, dv0):
-        return head_on_loss(dx0, dv0, n_each=n_each, k=20.0, precision=FP64, theta=0.0,
+        return head_on_loss(dx0, dv0, n_each=n_each, k=2.0, precision=FP64, theta=0.0,
                             separation=2.0, dts=dts)
 
     fd = fd_gradient(f, [0.0, 0.0], h=1e-5)
@@ -158,7 +159,8 @@ def test_density_vjp_matches_torch_autograd(ctx):
     p = ic.head_on(1500, eta=1.18, separation=2.0)
     s = SPHSystem(p, ctx, eta=1.18, eps=0.01, precision=FP64, n_group=32)
     s.initialize()
-    s.kdk(dt=DT)                     # moving particles; lists of the final positions
+    s.kdk(dt=DT)                     # moving particles
+    s.converge_density(full_lists=True)   # gather lists of the final positions, serving every particle
     w = s.lists["dens"]
     m = s.s["m"]
     inp = {k: s.s[k].detach().clone().requires_grad_(True) for k in _DENS_IN}
