@@ -66,8 +66,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --n-group for all kernels instead of tuning 8/16/32/64 per kernel")
     ap.add_argument("--groups", default=None,
-                    help="walk group sizes dens,hydro,grav[,density list capacity[,density members in "
-                         "registers 0/1]] (e.g. 32,32,32,256,1); implies --no-autotune")
+                    help="walk group sizes dens,hydro,grav[,density list capacity] (e.g. 32,32,32,256); "
+                         "implies --no-autotune")
     ap.add_argument("--no-kdk", action="store_true",
                     help="skip the full KDK step timing after the force step (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -119,7 +119,7 @@ def main() -> int:
                      precision=MIXED)
     if args.groups and len(args.groups.split(",")) > 3:
         gv = args.groups.split(",")
-        ctx.set_params(list_cap=int(gv[3]), dens_members_in_registers=int(gv[4]) if len(gv) > 4 else 0)
+        ctx.set_params(list_cap=int(gv[3]))
     t0 = time.perf_counter()
     rounds = sysm.relax_h(3)
     sysm.eos_balsara()
@@ -246,7 +246,6 @@ def main() -> int:
         "autotune_ms": None if tune is None else {k: {str(g): round(t, 4) for g, t in v.items()}
                                                   for k, v in tune[1].items()},
         "dens_list_cap": sysm.ctx.params["list_cap"] or 256,
-        "dens_members_in_registers": int(sysm.ctx.params["dens_members_in_registers"]),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dropin_fdps_path": dropin,

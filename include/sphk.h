@@ -82,11 +82,6 @@ typedef struct {
      * tiles.  192 needs less LDS (more waves per CU), 256 keeps more walks
      * resident.  0 (default): 256. */
     int list_cap;
-    /* Density functor, resident lists: 0 (default) sweeps each i-particle's
-     * compacted member indices from LDS in the three h-passes; 1 holds the
-     * members' exact distances and masses in registers, 8 per lane, and
-     * sweeps those.  Same results to rounding (np, itr and H bit for bit). */
-    int dens_members_in_registers;
 } sphk_params;
 
 /* ---- inputs / outputs (all FP64 unless stated) ------------------------ */

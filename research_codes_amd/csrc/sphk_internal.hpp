@@ -10,8 +10,7 @@ namespace sphk {
 
 hipError_t launch_density(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_dens_src &pi, const sphk_dens_src &pj,
-                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap,
-                          bool members_in_registers);
+                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap);
 
 hipError_t launch_hydro(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                         const int32_t *j_idx, const sphk_hydro_src &pi, const sphk_hydro_src &pj,

@@ -644,285 +644,6 @@ __global__ __launch_bounds__(kWave) void density_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Density, resident lists, members in registers.  One sweep marks, per
-// i-particle of a block of <= 64, the records within R_i = max(rs_i, H_0,i)
-// (bit masks in LDS).  The block's particles are then taken in chunks: a
-// particle with c members gets ceil(c / kNR) lanes, each holding kNR of its
-// members' distances and masses in registers, and a chunk is as many
-// particles as fit in the 64 lanes.  The distance is the reference's
-// r = sqrt(r2) (separate IEEE operations, correctly rounded sqrt), so
-// q = r / H is its q bit for bit and no edge re-evaluation is needed.  The
-// three h-passes run on the registers; the grad-h pass reads each member's
-// records once.  A chunk whose H_k outgrows R_i falls back to full sweeps.
-// ---------------------------------------------------------------------------
-constexpr int kNR = 8;
-
-__device__ __forceinline__ double exact_r(double dx, double dy, double dz)
-{
-#pragma clang fp contract(off)
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    return __builtin_sqrt(r2);
-}
-
-template <bool MIXED, int CAP>
-__global__ __launch_bounds__(kWave) void density_kernel_rc(
-    int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
-    const int32_t *__restrict__ j_idx, sphk_dens_src pi, sphk_dens_src pj, sphk_dens_out out,
-    double eta_ksrh, double hmax)
-{
-    using R = real_t<MIXED>;
-    static_assert(CAP % kTile == 0 && CAP <= kDensCap, "list capacity");
-    constexpr int kWords = CAP / kTile;
-    constexpr int kPad = CAP;   // inert record after the list
-    constexpr int kRed = MIXED ? 2 * kWave : 5 * kWave;
-    __shared__ double4 sp[CAP + 1];     // x, y, z, m
-    __shared__ R4<R> sv[CAP + 1];       // vx, vy, vz, id
-    __shared__ unsigned long long smask[kWave * kWords];   // member bits, [particle][word]
-    __shared__ uint16_t sk[kNR * kWave];                    // member records, [slot][lane]
-    __shared__ double red[kRed];
-    __shared__ double shs[kWave];
-    __shared__ int s_own[kWave];
-    __shared__ int s_cnt[kWave];
-
-    if ((int)blockIdx.x >= n_walk) return;
-    const int w = xcd_walk(blockIdx.x, n_walk);
-    const int lane = threadIdx.x;
-    const int ib = i_off[w], ie = i_off[w + 1];
-    const int jb = j_off[w], je = j_off[w + 1];
-    const int nj = je - jb;
-    const int ntile = (nj + kTile - 1) / kTile;
-
-    if (nj > CAP) {
-        for (int c0 = ib; c0 < ie; c0 += kWave) {
-            const Map M(min(kWave, ie - c0), lane);
-            density_streaming<MIXED>(lane, M, c0 + M.il, jb, je, j_idx, pi, pj, out, eta_ksrh, hmax,
-                                     sp, sv, red, shs);
-        }
-        return;
-    }
-
-    for (int e = lane; e < ntile * kTile; e += kWave) dens_stage<MIXED>(e, jb, je, j_idx, pj, sp, sv, true);
-    if (lane == 0) dens_stage<MIXED>(kPad, 0, 0, j_idx, pj, sp, sv, true);
-
-    for (int b0 = ib; b0 < ie; b0 += kWave) {
-        const int nb = min(kWave, ie - b0);
-        const Map M(nb, lane);
-        __syncthreads();   // staging done; LDS of the previous block no longer read
-        for (int q = lane; q < kWave * kWords; q += kWave) smask[q] = 0ull;
-        __syncthreads();
-
-        // -- membership sweep: lane (i, slice) marks its records within R_i
-        {
-            const int i = b0 + M.il;
-            const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
-            const double h0 = pi.ksr[i], rs = pi.rs[i];
-            const double rme = (h0 > rs ? h0 : rs) * (1. + 1e-12);
-            const double rm2 = rme * rme;
-#pragma unroll
-            for (int t = 0; t < kWords; t++) {
-                uint64_t b = 0;
-                if (t < ntile) {
-                    const double4 *tp = sp + t * kTile;
-                    const int nt = min(kTile, nj - t * kTile);
-                    int k = M.slice;
-                    for (; k + 3 * M.S < nt; k += 4 * M.S) {
-                        const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
-                        const double x0 = px - j0.x, y0 = py - j0.y, z0 = pz - j0.z;
-                        const double x1 = px - j1.x, y1 = py - j1.y, z1 = pz - j1.z;
-                        const double x2 = px - j2.x, y2 = py - j2.y, z2 = pz - j2.z;
-                        const double x3 = px - j3.x, y3 = py - j3.y, z3 = pz - j3.z;
-                        b |= (uint64_t)(x0 * x0 + y0 * y0 + z0 * z0 < rm2) << k;
-                        b |= (uint64_t)(x1 * x1 + y1 * y1 + z1 * z1 < rm2) << (k + M.S);
-                        b |= (uint64_t)(x2 * x2 + y2 * y2 + z2 * z2 < rm2) << (k + 2 * M.S);
-                        b |= (uint64_t)(x3 * x3 + y3 * y3 + z3 * z3 < rm2) << (k + 3 * M.S);
-                    }
-                    for (; k < nt; k += M.S) {
-                        const double dx = px - tp[k].x, dy = py - tp[k].y, dz = pz - tp[k].z;
-                        b |= (uint64_t)(dx * dx + dy * dy + dz * dz < rm2) << k;
-                    }
-                }
-                if (M.slice < M.S && b) atomicOr(&smask[M.il * kWords + t], (unsigned long long)b);
-            }
-        }
-        __syncthreads();
-        if (lane < nb) {
-            int c = 0;
-#pragma unroll
-            for (int t = 0; t < kWords; t++) c += __popcll(smask[lane * kWords + t]);
-            s_cnt[lane] = c;
-        }
-        __syncthreads();
-
-        for (int p0 = 0; p0 < nb;) {
-            // lanes per particle: ceil(c / kNR); take the leading particles that fit
-            // (a particle alone always fits: c <= CAP <= 32 kNR)
-            const int q = p0 + lane;
-            const int cq = q < nb ? s_cnt[q] : 0;
-            const int lq = q < nb ? max(1, (cq + kNR - 1) / kNR) : 0;
-            const int incl = wave_scan_incl(lq, lane);
-            const bool fits = q < nb && incl <= kWave;
-            const int n_take = __popcll(__ballot(fits));
-            s_own[lane] = -1;
-            __syncthreads();
-            if (fits)
-                for (int s2 = 0; s2 < lq; s2++) s_own[incl - lq + s2] = lane | (s2 << 8);
-            __syncthreads();
-            const int own = s_own[lane];
-            const bool busy = own >= 0;
-            int pl = busy ? (own & 0xff) : 0;   // the lane's particle, chunk-relative
-            const int sub = busy ? (own >> 8) : 0;
-            const int cnt_p = s_cnt[p0 + pl];
-            const int lanes_p = max(1, (cnt_p + kNR - 1) / kNR);
-            const int e_lo = sub * kNR;
-            const int n_e = busy ? min(kNR, cnt_p - e_lo) : 0;
-            bool owner = busy && sub == 0;
-
-            // the lane's members: ranks e_lo .. e_lo + n_e - 1 of its particle's bits
-            {
-                const unsigned long long *mk = smask + (p0 + pl) * kWords;
-                int skip = e_lo, got = 0;
-                for (int t = 0; t < kWords && got < n_e; t++) {
-                    unsigned long long b = mk[t];
-                    const int pc = __popcll(b);
-                    if (skip >= pc) {
-                        skip -= pc;
-                        continue;
-                    }
-                    for (; skip > 0; skip--) b &= b - 1;
-                    while (b && got < n_e) {
-                        sk[got * kWave + lane] = (uint16_t)(t * kTile + __builtin_ctzll(b));
-                        b &= b - 1;
-                        got++;
-                    }
-                }
-                for (; got < kNR; got++) sk[got * kWave + lane] = (uint16_t)kPad;
-            }
-            DensI I = dens_load_i(pi, b0 + p0 + pl);
-            double rr[kNR], mm[kNR];
-#pragma unroll
-            for (int s = 0; s < kNR; s++) {
-                const double4 j = sp[sk[s * kWave + lane]];
-                rr[s] = exact_r(I.x - j.x, I.y - j.y, I.z - j.z);
-                mm[s] = j.w;
-            }
-            const Map MF(n_take, lane);   // full sweeps over the chunk's particles
-            bool compact = true;
-            double hs = I.ksr;
-            double dens = 0.;
-            long long np = 0;
-
-            // -- three fixed-point passes (hdr_density.hpp:264-302)
-            for (int rep = 0; rep < 3; rep++) {
-                const double hi = 1. / hs;
-                const double hi3 = hi * hi * hi;
-                double v[2];
-                if (compact) {
-                    double rh = 0.;
-                    int nq = 0;
-#pragma unroll
-                    for (int s = 0; s < kNR; s++) {
-                        const double qv = rr[s] * hi;
-                        rh += mm[s] * hi3 * kernel0th(qv);
-                        nq += qv < 1. ? 1 : 0;
-                    }
-                    __syncthreads();
-                    red[lane] = rh;
-                    red[kWave + lane] = (double)nq;
-                    __syncthreads();
-                    v[0] = red[lane];
-                    v[1] = red[kWave + lane];
-                    if (owner)
-                        for (int t = 1; t < lanes_p; t++) {
-                            v[0] += red[lane + t];
-                            v[1] += red[kWave + lane + t];
-                        }
-                } else {
-                    Pass1 a{0., 0};
-                    for (int t = 0; t < ntile; t++) {
-                        const double4 *tp = sp + t * kTile;
-                        const int nt = min(kTile, nj - t * kTile);
-                        for (int k = MF.slice; k < nt; k += MF.S) dens_pass1_pair(tp[k], I.x, I.y, I.z, hi, hi3, a);
-                    }
-                    v[0] = a.rh;
-                    v[1] = (double)a.nj;
-                    slice_reduce<2>(v, red, MF, lane);
-                }
-                int viol = 0;
-                if (owner) {
-                    dens = v[0];
-                    np = (long long)v[1];
-                    const double hnew = dens_next_h(eta_ksrh, I.m, dens, hmax);
-                    shs[pl] = hnew;
-                    viol = compact && hnew > (I.ksr > I.rs ? I.ksr : I.rs);
-                }
-                // an H beyond R_i leaves the member lists incomplete: full
-                // sweeps from here on, the lead lanes of MF take over
-                if (__syncthreads_or(viol)) {
-                    if (owner) {
-                        red[pl] = dens;
-                        red[kWave + pl] = (double)np;
-                    }
-                    __syncthreads();
-                    compact = false;
-                    pl = MF.il;
-                    owner = MF.lead;
-                    I = dens_load_i(pi, b0 + p0 + pl);
-                    dens = red[pl];
-                    np = (long long)red[kWave + pl];
-                }
-                hs = shs[pl];
-            }
-
-            // -- grad-h, divergence, curl with H_3 (hdr_density.hpp:304-349)
-            const double hi_d = 1. / hs;
-            const R hi = (R)hi_d;
-            const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
-            const R vx = (R)I.vx, vy = (R)I.vy, vz = (R)I.vz;
-            double acc[5] = {0., 0., 0., 0., 0.};
-            if (compact) {
-                Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
-#pragma unroll
-                for (int s = 0; s < kNR; s++) {
-                    const int k = sk[s * kWave + lane];
-                    dens_pass2_pair<R>(sp[k], sv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-                }
-                const double part[5] = {(double)a.g, (double)a.dv, (double)a.rx, (double)a.ry, (double)a.rz};
-                // per-i sums over the particle's lanes in FP64, two quantities per round
-                for (int q0 = 0; q0 < 5; q0 += 2) {
-                    const int nq = q0 + 1 < 5 ? 2 : 1;
-                    __syncthreads();
-                    for (int qq = 0; qq < nq; qq++) red[qq * kWave + lane] = part[q0 + qq];
-                    __syncthreads();
-                    if (owner)
-                        for (int qq = 0; qq < nq; qq++) {
-                            double s = red[qq * kWave + lane];
-                            for (int t = 1; t < lanes_p; t++) s += red[qq * kWave + lane + t];
-                            acc[q0 + qq] = s;
-                        }
-                }
-            } else {
-                for (int t = 0; t < ntile; t++) {
-                    const double4 *tp = sp + t * kTile;
-                    const R4<R> *tv = sv + t * kTile;
-                    Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
-                    const int nt = min(kTile, nj - t * kTile);
-                    for (int k = MF.slice; k < nt; k += MF.S)
-                        dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-                    acc[0] += (double)a.g;
-                    acc[1] += (double)a.dv;
-                    acc[2] += (double)a.rx;
-                    acc[3] += (double)a.ry;
-                    acc[4] += (double)a.rz;
-                }
-                dens_reduce_grad<MIXED>(acc, red, MF, lane);
-            }
-            if (owner) dens_finish(out, b0 + p0 + pl, acc, dens, np, hs, I.rs);
-            p0 += n_take;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Hydro: scalar calcHydro, sph_yamauchi/run/hdr_hydro.hpp:320-421.
 // ---------------------------------------------------------------------------
 // Position record of a hydro j-entry: FP64 absolute in the FP64 path; in the
@@ -1386,19 +1107,12 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
 // ---------------------------------------------------------------------------
 hipError_t launch_density(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_dens_src &pi, const sphk_dens_src &pj,
-                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap,
-                          bool members_in_registers)
+                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap)
 {
     if (n_walk <= 0) return hipSuccess;
-#define SPHK_LAUNCH_D(M, CAPV)                                                                                    \
-    do {                                                                                                          \
-        if (members_in_registers)                                                                                 \
-            hipLaunchKernelGGL((density_kernel_rc<M, CAPV>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,    \
-                               j_off, j_idx, pi, pj, out, eta_ksrh, hmax);                                        \
-        else                                                                                                      \
-            hipLaunchKernelGGL((density_kernel<M, CAPV>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, \
-                               j_idx, pi, pj, out, eta_ksrh, hmax);                                               \
-    } while (0)
+#define SPHK_LAUNCH_D(M, CAPV)                                                                                     \
+    hipLaunchKernelGGL((density_kernel<M, CAPV>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, j_idx, \
+                       pi, pj, out, eta_ksrh, hmax)
     if (list_cap == 192) {
         if (mixed)
             SPHK_LAUNCH_D(true, 192);

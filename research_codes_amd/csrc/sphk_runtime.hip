@@ -136,7 +136,7 @@ struct TimeRec {
 struct sphk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    sphk_params params{1.6, 1.936492, 1e30, 0, 0, 0};
+    sphk_params params{1.6, 1.936492, 1e30, 0, 0};
     sphk::TagSlot tags[SPHK_TAG_MAX];
     bool timing = false;
     std::vector<sphk::TimeRec> pending;
@@ -378,9 +378,6 @@ int sphk_set_params(sphk_ctx *ctx, const sphk_params *p)
                          p->ksrh, p->hmax);
     if (p->list_cap != 0 && p->list_cap != 192 && p->list_cap != 256)
         return set_error(SPHK_E_ARG, "list_cap must be 0, 192 or 256 (got %d)", p->list_cap);
-    if (p->dens_members_in_registers != 0 && p->dens_members_in_registers != 1)
-        return set_error(SPHK_E_ARG, "dens_members_in_registers must be 0 or 1 (got %d)",
-                         p->dens_members_in_registers);
     ctx->params = *p;
     return SPHK_OK;
 }
@@ -423,8 +420,7 @@ int sphk_density_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
     Timer t(ctx, 0, st);
     SPHK_HIP(launch_density(st, n_walk, i_off, j_off, j_idx, pi, pj, out,
                             ctx->params.eta * ctx->params.ksrh, ctx->params.hmax,
-                            precision == SPHK_MIXED, ctx->params.list_cap == 192 ? 192 : 256,
-                            ctx->params.dens_members_in_registers != 0));
+                            precision == SPHK_MIXED, ctx->params.list_cap == 192 ? 192 : 256));
     t.stop();
     return SPHK_OK;
 }

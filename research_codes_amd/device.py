@@ -135,8 +135,7 @@ class Context:
         except Exception:
             pass
 
-    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None, list_cap=None,
-                   dens_members_in_registers=None):
+    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None, list_cap=None):
         p = L.Params()
         if getattr(self, "_params", None) is not None:
             p = self._params
@@ -150,16 +149,13 @@ class Context:
             p.uniform_softening = int(bool(uniform_softening))
         if list_cap is not None:
             p.list_cap = int(list_cap)
-        if dens_members_in_registers is not None:
-            p.dens_members_in_registers = int(bool(dens_members_in_registers))
         check(lib().sphk_set_params(self._h, C.byref(p)), "sphk_set_params")
         self._params = p
 
     @property
     def params(self):
         return dict(eta=self._params.eta, ksrh=self._params.ksrh, hmax=self._params.hmax,
-                    uniform_softening=bool(self._params.uniform_softening), list_cap=self._params.list_cap,
-                    dens_members_in_registers=bool(self._params.dens_members_in_registers))
+                    uniform_softening=bool(self._params.uniform_softening), list_cap=self._params.list_cap)
 
     @staticmethod
     def _stream():

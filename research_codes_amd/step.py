@@ -277,19 +277,17 @@ class SPHSystem:
             for g in candidates:
                 self.n_group[kind] = g
                 self.build_lists((kind,))
-                if kind == "dens":   # the density kernel's list capacity and member layout too
+                if kind == "dens":   # the density kernel's list capacity too
                     for cap in (192, 256):
-                        for rc in (0, 1):
-                            self.ctx.set_params(list_cap=cap, dens_members_in_registers=rc)
-                            table[kind][f"{g}/{cap}/{rc}"] = timed(run[kind])
+                        self.ctx.set_params(list_cap=cap)
+                        table[kind][f"{g}/{cap}"] = timed(run[kind])
                 else:
                     table[kind][str(g)] = timed(run[kind])
             key = min(table[kind], key=table[kind].get)
             if kind == "dens":
-                g, cap, rc = (int(v) for v in key.split("/"))
-                self.ctx.set_params(list_cap=cap, dens_members_in_registers=rc)
+                g, cap = (int(v) for v in key.split("/"))
+                self.ctx.set_params(list_cap=cap)
                 best["dens_list_cap"] = cap
-                best["dens_members_in_registers"] = rc
             else:
                 g = int(key)
             best[kind] = g

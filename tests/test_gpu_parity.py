@@ -37,12 +37,11 @@ def case(request):
     return polytrope_case(N, seed=1000 + N, n_group=g)
 
 
-@pytest.mark.parametrize("rc", [0, 1], ids=["idx", "regs"])
 @pytest.mark.parametrize("cap", [256, 192], ids=["cap256", "cap192"])
 @pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
-def test_density(ctx, case, prec, cap, rc):
+def test_density(ctx, case, prec, cap):
     p = case["p"]
-    ctx.set_params(eta=case["eta"], hmax=case["hmax"], list_cap=cap, dens_members_in_registers=rc)
+    ctx.set_params(eta=case["eta"], hmax=case["hmax"], list_cap=cap)
     w = D.Walks(case["i_off"], *case["dens_lists"])
     src = _dev(p, D.L.DENS_SRC)
     out = D.empty_dens_out(len(p["m"]))
@@ -50,7 +49,7 @@ def test_density(ctx, case, prec, cap, rc):
         ctx.density(w, src, src, out, prec)
         torch.cuda.synchronize()
     finally:
-        ctx.set_params(list_cap=0, dens_members_in_registers=0)
+        ctx.set_params(list_cap=0)
     ref = case["dens_out"]
     got = {k: v.cpu().numpy() for k, v in out.items()}
     assert np.array_equal(got["np"], ref["np"]), "neighbour counts differ"
@@ -154,22 +153,12 @@ def _density_vs_oracle(ctx, p, i_off, j_off, j_idx, eta, hmax, prec, label):
     return n_itr
 
 
-@pytest.mark.parametrize("rc", [0, 1], ids=["idx", "regs"])
 @pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
-def test_density_compaction_paths(ctx, prec, rc):
-    """The compacted density sweeps (both member layouts) against the oracle
-    where they switch paths: a search radius equal to H (many H_k exceed it:
-    full-sweep fallback, itr set), a wide radius (compacted lists overflow
-    one chunk), and a walk of 125 particles whose list fits in LDS (i-chunks
-    > 64)."""
-    ctx.set_params(dens_members_in_registers=rc)
-    try:
-        _compaction_paths(ctx, prec)
-    finally:
-        ctx.set_params(dens_members_in_registers=0)
-
-
-def _compaction_paths(ctx, prec):
+def test_density_compaction_paths(ctx, prec):
+    """The compacted density sweeps against the oracle where they switch
+    paths: a search radius equal to H (many H_k exceed it: full-sweep
+    fallback, itr set), a wide radius (compacted lists overflow one chunk),
+    and a walk of 125 particles whose list fits in LDS (i-chunks > 64)."""
     from research_codes_amd.tree import HostTree, sort_particles
     p = ic.polytrope(1.5, 3000, 1.0, 1.0, seed=42, eta=1.6)
     p, _ = sort_particles(p)
@@ -189,6 +178,66 @@ def _compaction_paths(ctx, prec):
            "ksr": np.full(n, 0.21), "rs": np.full(n, 0.3)}
     _density_vs_oracle(ctx, lat, np.array([0, n], np.int32), np.array([0, n], np.int32),
                        np.arange(n, dtype=np.int32), 1.6, 10.0, prec, "lattice125")
+
+
+def _nearby_counts(q, i_off, j_off, j_idx):
+    """List length of every walk, and for every particle the records of its
+    walk's list within max(ksr, rs) (its compacted member count)."""
+    R = np.maximum(q["ksr"], q["rs"])
+    cnt = np.zeros(len(q["m"]), np.int64)
+    for w in range(len(i_off) - 1):
+        js = j_idx[j_off[w]:j_off[w + 1]]
+        P = np.stack([q["x"][js], q["y"][js], q["z"][js]], 1)
+        for i in range(i_off[w], i_off[w + 1]):
+            d2 = ((P - np.array([q["x"][i], q["y"][i], q["z"][i]])) ** 2).sum(1)
+            cnt[i] = int((d2 < R[i] ** 2).sum())
+    return np.diff(j_off), cnt
+
+
+# search radius / H and walk group size of each long-list case
+LONG_PATHS = {"long": (1.25, 64), "long_h_outgrows": (1.0, 64),
+              "long_many_members": (2.2, 8), "beyond_compaction_buffer": (3.2, 32)}
+
+
+@pytest.mark.parametrize("cap", [192, 256], ids=["cap192", "cap256"])
+@pytest.mark.parametrize("path", list(LONG_PATHS))
+@pytest.mark.parametrize("prec", [D.FP64, D.MIXED], ids=["fp64", "mixed"])
+def test_density_long_lists(ctx, prec, path, cap):
+    """Lists longer than the LDS capacity stream through 64-record tiles in
+    every sweep: walks of 64 with a 1.25 H search radius; the same with
+    radius H, where H_k outgrows it (itr set); particles with more members
+    than the capacity; lists longer than the resident compaction buffer
+    (1728 records)."""
+    from research_codes_amd.tree import HostTree, sort_particles
+    fac, g = LONG_PATHS[path]
+    p = ic.polytrope(1.5, 4000, 1.0, 1.0, seed=77, eta=ic.ETA_50)
+    p, _ = sort_particles(p)
+    q = dict(p)
+    q["rs"] = fac * p["ksr"]
+    t = HostTree(q["x"], q["y"], q["z"], q["m"], None, 8, g)
+    i_off = t.i_off.copy()
+    j_off, j_idx = t.gather(q["rs"])
+    t.close()
+    nj, cnt = _nearby_counts(q, i_off, j_off, j_idx)
+    long_walks = np.nonzero((nj > cap) & (nj <= 1728))[0]
+    print(f"[{path} cap {cap}] walks {len(nj)}, list > cap {len(long_walks)}, > 1728 {(nj > 1728).sum()}, "
+          f"nearby max {cnt.max()}", flush=True)
+    if path == "beyond_compaction_buffer":
+        assert (nj > 1728).any()
+    else:
+        assert len(long_walks) > 0
+        in_long = np.concatenate([np.arange(i_off[w], i_off[w + 1]) for w in long_walks])
+        if path == "long_many_members":
+            assert (cnt[in_long] > cap).any()
+        else:
+            assert (cnt[in_long] <= cap).any()
+    ctx.set_params(list_cap=cap)
+    try:
+        n_itr = _density_vs_oracle(ctx, q, i_off, j_off, j_idx, ic.ETA_50, 10.0, prec, path)
+    finally:
+        ctx.set_params(list_cap=0)
+    if path == "long_h_outgrows":
+        assert n_itr > 0
 
 
 def test_list_cap_parameter_is_validated(ctx):
