@@ -14,10 +14,11 @@
 //      contraction off, so the FP64 sums round as on the host.
 //   4. Groups: a node is a walk if it holds <= n_group particles (or is a
 //      leaf) and its parent holds more; walks are ordered by first particle.
-//   5. Lists: the host's depth-first traversal of each walk is split at a
-//      frontier of subtrees (one tree level plus the leaves above it), one
-//      thread per (walk, subtree), run twice: count, then fill at scanned
-//      offsets; the output is the host's, element for element.
+//   5. Lists: the host's depth-first traversal of each walk, 8 lanes per
+//      walk (the children of one opened node at a time, a stack of pending
+//      entries in LDS), run twice: count, then fill at scanned offsets; the
+//      output is the host's, element for element.  The short-range lists test
+//      each reached leaf's particles for exact membership on the way.
 #include <hip/hip_runtime.h>
 
 #include <cstring>   // rocPRIM's iterator headers use memset without including it
@@ -592,194 +593,213 @@ __device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&
     return m.child < 0 ? kLeaf : kOpen;
 }
 
-// One lane's output stream of int32 entries, written in aligned 16-byte
-// pieces: entries before the first multiple-of-4 position go out one by one,
-// then four at a time from registers, the rest at flush().  The long list's
-// entries arrive one per tree visit; single 4-byte stores from 64 walks at
-// once leave partly written cache lines that the memory side completes one
-// sector per entry.
-struct Out4 {
-    int32_t *out;
-    int64_t pos;   // next position; entries [pos - n, pos) wait in b0..b3
-    int n;
-    int32_t b0, b1, b2, b3;
-    __device__ __forceinline__ void put(int32_t v)
-    {
-        if (n == 0 && (pos & 3) != 0) {
-            out[pos++] = v;
-            return;
-        }
-        b0 = n == 0 ? v : b0;
-        b1 = n == 1 ? v : b1;
-        b2 = n == 2 ? v : b2;
-        b3 = n == 3 ? v : b3;
-        n++;
-        pos++;
-        if (n == 4) {
-            *reinterpret_cast<int4 *>(out + pos - 4) = make_int4(b0, b1, b2, b3);
-            n = 0;
-        }
-    }
-    __device__ __forceinline__ void flush()
-    {
-        const int64_t p = pos - n;
-        if (n > 0) out[p] = b0;
-        if (n > 1) out[p + 1] = b1;
-        if (n > 2) out[p + 2] = b2;
-        n = 0;
-    }
-};
+// ---- interaction lists: 8 lanes per walk -----------------------------------
+// The host builder's depth-first walk of each group, 8 walks per wave and 8
+// lanes per walk (an octree node has at most 8 children).  A walk keeps a
+// stack of evaluated entries in LDS, the next one in preorder on top.  Each
+// step evaluates the children of one opened node, a lane each: the finished
+// children (SPJ, leaf) before the first child to open are emitted at once,
+// the rest are pushed, first child on top.  Then the finished entries on top
+// of the stack are popped and emitted, up to 8 at a time, down to the next
+// node to open, whose children the next step evaluates.  Emitted in this
+// order, the entries are the host's list element for element.  Each tree
+// level leaves at most 7 siblings pending, so the stack never holds more
+// than 7 (kMaxLevel + 1) + 8 entries.
+//
+// Entry: word 0 the node (SPJ, and leaves in candidate-leaf mode), the first
+// child (to open) or the first particle (leaf, particles listed); word 1 the
+// type in bits 29-31 and the child or particle count below.
+constexpr int kListWalks = 8;
+constexpr int kListStack = 7 * (kMaxLevel + 1) + 8 + 2;   // 164 (even: 8-byte rows)
+constexpr uint32_t kTySpj = 1u << 29, kTyLeaf = 2u << 29, kTyOpen = 3u << 29, kTyMask = 7u << 29;
+constexpr uint32_t kCountMask = (1u << 29) - 1;
 
-// The host builder's depth-first walk of one group, split at the frontier:
-// frontier node f (the nodes of one level plus the leaves above it, in
-// particle order) gets its own thread per walk.  The thread first replays
-// the walk's decisions on the ancestors of f, root first: if one is skipped
-// nothing is emitted; if one is taken as an SPJ it is emitted by the thread
-// of its first frontier node only.  Otherwise it walks the subtree of f.
-// Concatenated in frontier order, the threads' outputs are the host's list.
-// MODE 0 gather, 1 symmetry, 2 long; with FILL entries are written from the
-// scanned offsets, else counted.
-template <int MODE, bool FILL, bool LEAVES = false>
-__global__ __launch_bounds__(64) void list_kernel(int ng, int nf, TreeView t, const int32_t *__restrict__ front,
-                                                  const double *__restrict__ wb, const double *__restrict__ r,
-                                                  double inv_t2, int64_t *__restrict__ cj,
-                                                  int64_t *__restrict__ cs, const int64_t *__restrict__ oj,
-                                                  const int64_t *__restrict__ os, int32_t *__restrict__ jout,
-                                                  int32_t *__restrict__ sout)
+// bits of the 8 lanes of this lane's walk
+__device__ __forceinline__ unsigned walk_bits(bool p, int lane)
+{
+    return (unsigned)((__ballot(p) >> (lane & ~7)) & 0xffu);
+}
+
+__device__ __forceinline__ int walk_scan_incl(int v, int c)
+{
+    for (int d = 1; d < 8; d <<= 1) {
+        const int u = __shfl_up(v, d, 8);
+        if (c >= d) v += u;
+    }
+    return v;
+}
+
+// Exact membership of particle j of a reached leaf in a short-range list,
+// as the host's leaf loop decides it: MODE 0 |x_j - x_i| < rs_i for some i of
+// the walk; MODE 1 r_ij < max(H_i, H_j) for some i, after the box pre-tests
+// (j inside the walk's search box, or j's sphere reaching the group's box).
+// The walk's i-particles (x, y, z, r) are read from si, the first kWalkI of
+// them staged in LDS.
+constexpr int kWalkI = 256;
+
+template <int MODE>
+__device__ __forceinline__ bool member(const TreeView &t, const double *__restrict__ r, int j, int ib, int ni,
+                                       const double4 *si, const double (&olo)[3], const double (&ohi)[3],
+                                       const double (&glo)[3], const double (&ghi)[3])
 {
 #pragma clang fp contract(off)
-    // neighbouring lanes take neighbouring walks under the same frontier
-    // node: their traversals follow nearly the same path
-    const int64_t t0 = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (t0 >= (int64_t)ng * nf) return;
-    const int g = (int)(t0 % ng), k = (int)(t0 / ng);
-    const int64_t tid = (int64_t)g * nf + k;   // slot of (walk, frontier node) in the count arrays
-    const int f = front[k];
-    const size_t C = t.mo.cap;
-    const int ib = t.i_off[g], ie = t.i_off[g + 1];
-    const int gn = t.group[g];
-    double olo[3] = {0., 0., 0.}, ohi[3] = {0., 0., 0.}, glo[3], ghi[3];
+    const double p[3] = {t.x[j], t.y[j], t.z[j]};
+    const double hj = MODE == 1 ? r[j] : 0.;
+    bool in = true, jov = MODE == 1;
     for (int d = 0; d < 3; d++) {
-        if (MODE != 2) {
-            olo[d] = wb[d * ng + g];
-            ohi[d] = wb[(3 + d) * ng + g];
-        }
-        glo[d] = t.mo.lo[d * C + gn];
-        ghi[d] = t.mo.hi[d * C + gn];
+        in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
+        if (MODE == 1) jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
     }
-    int64_t nj = 0, ns = 0;
-    const int64_t j0 = FILL ? oj[tid] : 0;
-    const int64_t s0 = (FILL && MODE == 2) ? os[tid] : 0;
-    // long list: both streams through write-combining buffers
-    Out4 jw{jout, j0, 0, 0, 0, 0, 0}, sw{sout, s0, 0, 0, 0, 0, 0};
+    if (!in && !jov) return false;
+    auto hit = [&](double4 P) {
+        const double dx = p[0] - P.x, dy = p[1] - P.y, dz = p[2] - P.z;
+        const double rr = MODE == 1 ? (P.w > hj ? P.w : hj) : P.w;
+        return dx * dx + dy * dy + dz * dz < rr * rr;
+    };
+    const int nl = ni < kWalkI ? ni : kWalkI;
+    int q = 0;
+    for (; q + 2 <= nl; q += 2) {   // LDS, two at a time
+        const double4 P0 = si[q], P1 = si[q + 1];
+        if (hit(P0) || hit(P1)) return true;
+    }
+    if (q < nl && hit(si[q])) return true;
+    for (q = nl; q < ni; q++) {   // beyond the staged particles: from memory
+        const int i = ib + q;
+        if (hit(make_double4(t.x[i], t.y[i], t.z[i], r[i]))) return true;
+    }
+    return false;
+}
 
-    // ancestors of f, root first (the frontier is a few levels deep)
-    const int lf = t.nd.level[f];
-    bool reach = true;
-    NodeLinks ln;
-    for (int L = 0; L < lf && reach; L++) {
-        int a = f;
-        for (int u = lf; u > L; u--) a = t.nd.parent[a];
-        const Visit v = visit<MODE>(t, a, olo, ohi, glo, ghi, inv_t2, ln);
-        if (v == kOpen) continue;
-        reach = false;
-        if (v == kSpj && ln.begin == t.nd.begin[f]) {
-            if (FILL) sw.put(a);
-            ns++;
+// The particles of the leaves reached into jout (MODE 0 gather, 1 symmetry:
+// the candidates of the short-range lists), and for MODE 2 (long) the SPJ
+// nodes into sout: with FILL from the walk's offsets oj/os, else the walk's
+// counts go to cj/cs.  err is set if a stack would overflow (a corrupt
+// tree).
+template <int MODE, bool FILL>
+__global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const double *__restrict__ wb,
+                                                       const double *__restrict__ r, double inv_t2,
+                                                       int64_t *__restrict__ cj, int64_t *__restrict__ cs,
+                                                       const int64_t *__restrict__ oj, const int64_t *__restrict__ os,
+                                                       int32_t *__restrict__ jout, int32_t *__restrict__ sout,
+                                                       int *__restrict__ err)
+{
+#pragma clang fp contract(off)
+    __shared__ uint2 stk[kListWalks][kListStack];
+    const int lane = threadIdx.x, sub = lane >> 3, c = lane & 7;
+    const int g = blockIdx.x * kListWalks + sub;
+    const bool live = g < ng;
+    const unsigned below = (1u << c) - 1u, above = 0xffu & ~((2u << c) - 1u);
+    const size_t C = t.mo.cap;
+    double olo[3] = {0., 0., 0.}, ohi[3] = {0., 0., 0.}, glo[3] = {0., 0., 0.}, ghi[3] = {0., 0., 0.};
+    int64_t nj = 0, ns = 0, j0 = 0, s0 = 0;
+    if (live) {
+        const int gn = t.group[g];
+        for (int d = 0; d < 3; d++) {
+            if (MODE != 2) {
+                olo[d] = wb[d * ng + g];
+                ohi[d] = wb[(3 + d) * ng + g];
+            }
+            glo[d] = t.mo.lo[d * C + gn];
+            ghi[d] = t.mo.hi[d * C + gn];
+        }
+        if (FILL) {
+            j0 = oj[g];
+            if (MODE == 2) s0 = os[g];
         }
     }
-    if (reach) {
-        // preorder over f's subtree by skip pointers: open = first child,
-        // otherwise continue at next (f's own next ends the subtree)
-        const int stop = MODE == 2 ? t.mr[f].next : t.br[f].next;
-        for (int nxt = f; nxt != stop;) {
-            const int a = nxt;
+    // emit the entries (w0, w1) of the lanes with em set, in lane order
+    auto emit = [&](bool em, uint32_t w0, uint32_t w1) {
+        const uint32_t ty = w1 & kTyMask;
+        if (MODE == 2) {
+            const bool is_s = em && ty == kTySpj;
+            const unsigned sb = walk_bits(is_s, lane);
+            if (FILL && is_s) sout[s0 + ns + __popc(sb & below)] = (int32_t)w0;
+            ns += __popc(sb);
+        }
+        const bool is_l = em && ty == kTyLeaf;
+        const int sz = is_l ? (int)(w1 & kCountMask) : 0;
+        const int incl = walk_scan_incl(sz, c);
+        if (FILL && is_l) {
+            const int64_t p0 = j0 + nj + incl - sz;
+            for (int q = 0; q < sz; q++) jout[p0 + q] = (int32_t)w0 + q;
+        }
+        nj += __shfl(incl, 7, 8);
+    };
+
+    int sp = 0;                        // the walk's stack depth
+    int first = 0, nch = live ? 1 : 0; // nodes evaluated by the next step: the root first
+    bool bad = false;
+    while (__ballot(nch > 0)) {
+        // evaluate nodes first + c
+        const bool ev = c < nch;
+        uint32_t w0 = 0, w1 = 0;
+        if (ev) {
+            const int a = first + c;
+            NodeLinks ln;
             const Visit v = visit<MODE>(t, a, olo, ohi, glo, ghi, inv_t2, ln);
-            if (v == kOpen) {
-                nxt = ln.child;
-                continue;
-            }
-            nxt = ln.next;
-            if (v == kSkip) continue;
             if (v == kSpj) {
-                if (FILL) sw.put(a);
-                ns++;
-                continue;
-            }
-            if (LEAVES) {   // candidate leaf; its particles are filtered by member_kernel
-                if (FILL) jout[j0 + nj] = a;
-                nj++;
-                continue;
-            }
-            // leaf: its particles, filtered by exact membership for the short lists
-            for (int j = ln.begin; j < ln.end; j++) {
-                if (MODE == 2) {
-                    if (FILL) jw.put(j);
-                    nj++;
-                    continue;
-                }
-                const double p[3] = {t.x[j], t.y[j], t.z[j]};
-                const double hj = MODE == 1 ? r[j] : 0.;
-                bool in = true, jov = MODE == 1;
-                for (int d = 0; d < 3; d++) {
-                    in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
-                    if (MODE == 1) jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
-                }
-                if (!in && !jov) continue;
-                for (int i = ib; i < ie; i++) {
-                    const double dx = p[0] - t.x[i], dy = p[1] - t.y[i], dz = p[2] - t.z[i];
-                    const double rr = MODE == 1 ? (r[i] > hj ? r[i] : hj) : r[i];
-                    if (dx * dx + dy * dy + dz * dz < rr * rr) {
-                        if (FILL) jout[j0 + nj] = j;
-                        nj++;
-                        break;
-                    }
-                }
+                w0 = (uint32_t)a;
+                w1 = kTySpj;
+            } else if (v == kLeaf) {
+                w0 = (uint32_t)ln.begin;
+                w1 = kTyLeaf | (uint32_t)(ln.end - ln.begin);
+            } else if (v == kOpen) {
+                w0 = (uint32_t)ln.child;
+                w1 = kTyOpen | (uint32_t)t.nd.nchild[a];
             }
         }
+        const bool kept = ev && w1 != 0;
+        const unsigned ob = walk_bits(kept && (w1 & kTyMask) == kTyOpen, lane);
+        const int fo = ob ? __builtin_ctz(ob) : 8;
+        emit(kept && c < fo, w0, w1);
+        const bool push = kept && c >= fo;
+        const unsigned pb = walk_bits(push, lane);
+        if (sp + __popc(pb) > kListStack) bad = true;
+        if (push && !bad) stk[sub][sp + __popc(pb & above)] = make_uint2(w0, w1);
+        if (!bad) sp += __popc(pb);
+        // pop finished entries down to the next node to open
+        nch = 0;
+        while (sp > 0 && !bad) {
+            const int avail = sp < 8 ? sp : 8;
+            const uint2 e = c < avail ? stk[sub][sp - 1 - c] : make_uint2(0u, 0u);
+            const unsigned ob2 = walk_bits(c < avail && (e.y & kTyMask) == kTyOpen, lane);
+            const int k = ob2 ? __builtin_ctz(ob2) : avail;
+            emit(c < k, e.x, e.y);
+            if (ob2) {
+                first = (int)__shfl((int)e.x, k, 8);
+                nch = (int)(__shfl((int)e.y, k, 8) & kCountMask);
+                sp -= k + 1;
+                break;
+            }
+            sp -= k;
+        }
     }
-    if (FILL && MODE == 2) {
-        jw.flush();
-        sw.flush();
-    }
-    if (!FILL) {
-        cj[tid] = nj;
-        if (MODE == 2) cs[tid] = ns;
+    if (bad && c == 0) atomicOr(err, 1);
+    if (!FILL && live && c == 0) {
+        cj[g] = nj;
+        if (MODE == 2) cs[g] = ns;
     }
 }
 
-// Exact membership of the particles of each walk's candidate leaves, one
-// wave per walk: the walk's i-particles sit in LDS, lane l tests candidate
-// c0 + l of the walk's leaves laid end to end (the host's order), and a
-// ballot orders the accepted ones.  MODE 0: |x_j - x_i| < rs_i for some i;
-// MODE 1: r_ij < max(H_i, H_j) for some i (after the same box pre-tests as
-// the host's leaf loop).  The accepted j are written from off[g] (room for
-// all of the walk's candidates) and their number to cnt[g].
-constexpr int kMemberCap = 256;   // i-particles of a walk held in LDS
-
+// Exact membership of each walk's candidates (the particles of the leaves its
+// walk reached, in the host's order), one wave per walk: the walk's
+// i-particles sit in LDS, lane l tests candidate c0 + l, and the members are
+// compacted in place (in order) at the front of the walk's stretch of scr;
+// their number goes to cnt.
 template <int MODE>
-__global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const int32_t *__restrict__ lo,
-                                                    const int32_t *__restrict__ li, const double *__restrict__ wb,
-                                                    const double *__restrict__ r, int64_t *__restrict__ cnt,
-                                                    const int64_t *__restrict__ off, int32_t *__restrict__ jout)
+__global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const double *__restrict__ wb,
+                                                    const double *__restrict__ r, const int64_t *__restrict__ coff,
+                                                    const int64_t *__restrict__ ccnt, int32_t *__restrict__ scr,
+                                                    int64_t *__restrict__ cnt)
 {
-#pragma clang fp contract(off)
-    __shared__ double si[4][kMemberCap];
-    __shared__ int32_t lbeg[64], lpre[65];
+    __shared__ double4 si[kWalkI];
     const int g = blockIdx.x;
     if (g >= ng) return;
     const int lane = threadIdx.x;
     const size_t C = t.mo.cap;
-    const int ib = t.i_off[g], ie = t.i_off[g + 1];
-    const int ni = ie - ib;
-    const bool in_lds = ni <= kMemberCap;
-    if (in_lds)
-        for (int q = lane; q < ni; q += 64) {
-            si[0][q] = t.x[ib + q];
-            si[1][q] = t.y[ib + q];
-            si[2][q] = t.z[ib + q];
-            si[3][q] = r[ib + q];
-        }
+    const int ib = t.i_off[g], ni = t.i_off[g + 1] - ib;
+    for (int q = lane; q < ni && q < kWalkI; q += 64)
+        si[q] = make_double4(t.x[ib + q], t.y[ib + q], t.z[ib + q], r[ib + q]);
     const int gn = t.group[g];
     double olo[3], ohi[3], glo[3], ghi[3];
     for (int d = 0; d < 3; d++) {
@@ -788,90 +808,24 @@ __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const in
         glo[d] = t.mo.lo[d * C + gn];
         ghi[d] = t.mo.hi[d * C + gn];
     }
+    __syncthreads();
+    int32_t *s = scr + coff[g];
+    const int64_t nc = ccnt[g];
     int64_t count = 0;
-    const int64_t base = off[g];
-    const int lb = lo[g], le = lo[g + 1];
-    for (int l0 = lb; l0 < le; l0 += 64) {
-        const int nl = min(64, le - l0);
-        __syncthreads();
-        int sz = 0;
-        if (lane < nl) {
-            const int a = li[l0 + lane];
-            lbeg[lane] = t.nd.begin[a];
-            sz = t.nd.end[a] - t.nd.begin[a];
-        }
-        // inclusive scan of the leaf sizes
-        int incl = sz;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
-        }
-        lpre[lane + 1] = incl;
-        if (lane == 0) lpre[0] = 0;
-        __syncthreads();
-        const int total = lpre[nl];
-        for (int c0 = 0; c0 < total; c0 += 64) {
-            const int c = c0 + lane;
-            bool pass = false;
-            int j = -1;
-            if (c < total) {
-                int lo_ = 0, hi_ = nl - 1;   // leaf q with lpre[q] <= c < lpre[q+1]
-                while (lo_ < hi_) {
-                    const int mid = (lo_ + hi_ + 1) >> 1;
-                    if (lpre[mid] <= c)
-                        lo_ = mid;
-                    else
-                        hi_ = mid - 1;
-                }
-                j = lbeg[lo_] + (c - lpre[lo_]);
-                const double p[3] = {t.x[j], t.y[j], t.z[j]};
-                const double hj = MODE == 1 ? r[j] : 0.;
-                bool in = true, jov = MODE == 1;
-                for (int d = 0; d < 3; d++) {
-                    in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
-                    if (MODE == 1) jov = jov && !(ghi[d] < p[d] - hj || glo[d] > p[d] + hj);
-                }
-                if (in || jov)
-                    for (int q = 0; q < ni; q++) {
-                        const double xi = in_lds ? si[0][q] : t.x[ib + q];
-                        const double yi = in_lds ? si[1][q] : t.y[ib + q];
-                        const double zi = in_lds ? si[2][q] : t.z[ib + q];
-                        const double ri = in_lds ? si[3][q] : r[ib + q];
-                        const double dx = p[0] - xi, dy = p[1] - yi, dz = p[2] - zi;
-                        const double rr = MODE == 1 ? (ri > hj ? ri : hj) : ri;
-                        if (dx * dx + dy * dy + dz * dz < rr * rr) {
-                            pass = true;
-                            break;
-                        }
-                    }
-            }
-            const uint64_t bal = __ballot(pass);
-            if (pass) jout[base + count + __popcll(bal & ((1ull << lane) - 1ull))] = j;
-            count += __popcll(bal);
-        }
+    for (int64_t c0 = 0; c0 < nc; c0 += 64) {
+        const int64_t e = c0 + lane;
+        const int j = e < nc ? s[e] : 0;
+        const bool acc = e < nc && member<MODE>(t, r, j, ib, ni, si, olo, ohi, glo, ghi);
+        const uint64_t bal = __ballot(acc);
+        // writes land at or before the chunk's own (already read) slots
+        if (acc) s[count + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+        count += __popcll(bal);
     }
     if (lane == 0) cnt[g] = count;
 }
 
-// Particles in each walk's candidate leaves (the room member_kernel may
-// need for the walk's accepted particles), one wave per walk.
-__global__ __launch_bounds__(64) void cand_total_kernel(int ng, const int32_t *__restrict__ lo,
-                                                        const int32_t *__restrict__ li, NodeArrays nd,
-                                                        int64_t *__restrict__ tot)
-{
-    const int g = blockIdx.x;
-    if (g >= ng) return;
-    int s = 0;
-    for (int l = lo[g] + (int)threadIdx.x; l < lo[g + 1]; l += 64) {
-        const int a = li[l];
-        s += nd.end[a] - nd.begin[a];
-    }
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (threadIdx.x == 0) tot[g] = s;
-}
-
-// Accepted particles of each walk, from its stretch of the member scratch
-// to its place in the list.
+// Members of each walk, from its stretch of the candidate scratch to its
+// place in the list.
 __global__ __launch_bounds__(64) void member_copy_kernel(int ng, const int64_t *__restrict__ soff,
                                                          const int32_t *__restrict__ j_off,
                                                          const int32_t *__restrict__ scr, int32_t *__restrict__ jout)
@@ -883,32 +837,14 @@ __global__ __launch_bounds__(64) void member_copy_kernel(int ng, const int64_t *
     for (int k = threadIdx.x; k < n; k += 64) jout[b + k] = scr[s0 + k];
 }
 
-// Frontier nodes: the nodes of level L and the leaves above it.
-__global__ __launch_bounds__(kBlock) void front_mark_kernel(int nn, NodeArrays nd, int L, int32_t *__restrict__ head)
-{
-    const int a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= nn) return;
-    const int lv = nd.level[a];
-    if (lv == L || (lv < L && nd.child[a] < 0)) head[nd.begin[a]] = a;
-}
-
-__global__ __launch_bounds__(kBlock) void front_write_kernel(int n, const int32_t *__restrict__ head,
-                                                             const int64_t *__restrict__ pos,
-                                                             int32_t *__restrict__ front)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n && head[i] >= 0) front[pos[i]] = head[i];
-}
-
-// Walk offsets from the scanned (walk, frontier) counts: off[g] = scan[g nf].
-__global__ void offsets_kernel(int ng, int nf, const int64_t *__restrict__ scan, const int64_t *__restrict__ cnt,
+// Walk offsets from the scanned per-walk counts, and the total.
+__global__ void offsets_kernel(int ng, const int64_t *__restrict__ scan, const int64_t *__restrict__ cnt,
                                int32_t *__restrict__ off, int64_t *__restrict__ total)
 {
     const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g < ng) off[g] = (int32_t)scan[(int64_t)g * nf];
+    if (g < ng) off[g] = (int32_t)scan[g];
     if (g == ng - 1) {
-        const int64_t last = (int64_t)ng * nf - 1;
-        const int64_t t = scan[last] + cnt[last];
+        const int64_t t = scan[g] + cnt[g];
         *total = t;
         off[ng] = (int32_t)t;
     }
@@ -925,8 +861,8 @@ struct sphk_dtree {
     std::vector<int> level_start;   // node index of the first node of each level
     int n_node = 0;
     DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
-    DArr<int32_t> head, group, i_off, front, leaf_off, leaf_idx, mscr, nextp;
-    int n_walk = 0, n_front = 0;
+    DArr<int32_t> head, group, i_off, mscr, nextp;
+    int n_walk = 0;
     DArr<double> wbox;
     DArr<BoxRec> brec;
     DArr<MacRec> mrec;
@@ -948,8 +884,7 @@ struct sphk_dtree {
                         &sz, &se, &olo, &ohi})
             a->release();
         key.release();
-        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &front, &leaf_off,
-                        &leaf_idx, &mscr, &nextp})
+        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &mscr, &nextp})
             a->release();
         wbox.release();
         brec.release();
@@ -1000,24 +935,52 @@ int read1(hipStream_t st, const T *dev, T *host)
 
 int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
-template <int MODE>
-int build_list_two_stage(sphk_dtree *t, const double *r);
+// Both totals of a list: two int64 written by offsets_kernel (one host read).
+int read_totals(sphk_dtree *t, int64_t (&tot)[2])
+{
+    int bad = 0;
+    SPHK_TRY(hipMemcpyAsync(tot, t->total.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, t->st));
+    SPHK_TRY(hipMemcpyAsync(&bad, t->err.p, sizeof(int), hipMemcpyDeviceToHost, t->st));
+    SPHK_TRY(hipStreamSynchronize(t->st));
+    if (bad) return set_error(SPHK_E_STATE, "dtree: list walk stack overflow (corrupt tree)");
+    return SPHK_OK;
+}
 
+// Per-walk counts -> offsets and totals: counts a (and b) are scanned into
+// c (and d); off_a / off_b receive the walk offsets.
+int walk_offsets(sphk_dtree *t, int ng, const int64_t *a, int64_t *c, int32_t *off_a, const int64_t *b,
+                 int64_t *d, int32_t *off_b)
+{
+    if (int rc = scan64(t, a, c, ng)) return rc;
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, c, a, off_a, t->total.p);
+    if (b) {
+        if (int rc = scan64(t, b, d, ng)) return rc;
+        hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, d, b, off_b, t->total.p + 1);
+    }
+    SPHK_TRY(hipGetLastError());
+    return SPHK_OK;
+}
+
+// Interaction lists of every walk.  Long list: the tree walk (count, scan,
+// fill) gives each walk's EPJ particles and SPJ nodes.  Short-range lists
+// (gather: r = rs, symmetry: r = H): the tree walk (count, scan, fill)
+// writes the particles of the leaves reached into the walk's stretch of a
+// scratch array, member_kernel keeps the members there, and they are copied
+// to their offsets in the list.
 template <int MODE>
 int build_list(sphk_dtree *t, const double *r, double inv_t2)
 {
-    if (MODE != 2) return build_list_two_stage<MODE>(t, r);
     DLists &L = t->lists[MODE];
     L.built = false;
-    const int ng = t->n_walk, nf = t->n_front;
-    const int64_t nt = (int64_t)ng * nf;
-    SPHK_TRY(t->i64a.reserve(nt, false, t->st));   // EPJ counts per (walk, frontier node)
-    SPHK_TRY(t->i64b.reserve(nt, false, t->st));   // SPJ counts
-    SPHK_TRY(t->i64c.reserve(nt, false, t->st));   // EPJ offsets
-    SPHK_TRY(t->i64d.reserve(nt, false, t->st));   // SPJ offsets
+    const int ng = t->n_walk;
+    SPHK_TRY(t->i64a.reserve(ng, false, t->st));   // EPJ counts per walk
+    SPHK_TRY(t->i64b.reserve(ng, false, t->st));   // SPJ counts
+    SPHK_TRY(t->i64c.reserve(ng, false, t->st));   // EPJ offsets
+    SPHK_TRY(t->i64d.reserve(ng, false, t->st));   // SPJ offsets
     SPHK_TRY(t->total.reserve(2, false, t->st));
     SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
     if (MODE == 2) SPHK_TRY(L.s_off.reserve(ng + 1, false, t->st));
+    SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
     if (MODE != 2) {
         SPHK_TRY(t->wbox.reserve(6 * (size_t)ng, false, t->st));
         hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p,
@@ -1025,103 +988,46 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
     }
     TreeView v{t->nodes(), t->moments(), t->brec.p, t->mrec.p, t->orec.p, t->x.p, t->y.p, t->z.p, t->group.p,
                t->i_off.p};
-    const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
-    hipLaunchKernelGGL((list_kernel<MODE, false>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
-                       t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr);
+    const double *wb = MODE == 2 ? nullptr : t->wbox.p;
+    const dim3 lg((unsigned)((ng + kListWalks - 1) / kListWalks)), lb(64);
+    hipLaunchKernelGGL((walk_list_kernel<MODE, false>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2, t->i64a.p,
+                       t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
     SPHK_TRY(hipGetLastError());
+    if (int rc = walk_offsets(t, ng, t->i64a.p, t->i64c.p, L.j_off.p, MODE == 2 ? t->i64b.p : nullptr, t->i64d.p,
+                              L.s_off.p))
+        return rc;
     int64_t tot[2] = {0, 0};
-    if (nt > std::numeric_limits<int>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many walk subtrees");
-    if (int rc = scan64(t, t->i64a.p, t->i64c.p, (int)nt)) return rc;
-    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64c.p, t->i64a.p,
-                       L.j_off.p, t->total.p);
-    if (int rc = read1(t->st, t->total.p, &tot[0])) return rc;
-    if (MODE == 2) {
-        if (int rc = scan64(t, t->i64b.p, t->i64d.p, (int)nt)) return rc;
-        hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64d.p, t->i64b.p,
-                           L.s_off.p, t->total.p + 1);
-        if (int rc = read1(t->st, t->total.p + 1, &tot[1])) return rc;
-    }
+    if (int rc = read_totals(t, tot)) return rc;
+    if (MODE != 2) tot[1] = 0;
     if (tot[0] > std::numeric_limits<int32_t>::max() || tot[1] > std::numeric_limits<int32_t>::max())
         return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
-    SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
-    if (MODE == 2) SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
-    hipLaunchKernelGGL((list_kernel<MODE, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, inv_t2,
-                       nullptr, nullptr, t->i64c.p, MODE == 2 ? t->i64d.p : nullptr, L.j_idx.p,
-                       MODE == 2 ? L.s_idx.p : nullptr);
-    SPHK_TRY(hipGetLastError());
+    if constexpr (MODE == 2) {
+        SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
+        SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
+        hipLaunchKernelGGL((walk_list_kernel<MODE, true>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2, nullptr, nullptr,
+                           t->i64c.p, t->i64d.p, L.j_idx.p, L.s_idx.p, t->err.p);
+        SPHK_TRY(hipGetLastError());
+    } else {
+        // tot[0] candidates into the scratch; the members compacted in place,
+        // their counts to i64b
+        SPHK_TRY(t->mscr.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
+        hipLaunchKernelGGL((walk_list_kernel<MODE, true>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2, nullptr, nullptr,
+                           t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
+        hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, wb, r, t->i64c.p, t->i64a.p,
+                           t->mscr.p, t->i64b.p);
+        SPHK_TRY(hipGetLastError());
+        if (int rc = walk_offsets(t, ng, t->i64b.p, t->i64d.p, L.j_off.p, nullptr, nullptr, nullptr)) return rc;
+        if (int rc = read_totals(t, tot)) return rc;
+        tot[1] = 0;
+        if (tot[0] > std::numeric_limits<int32_t>::max())
+            return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
+        SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
+        hipLaunchKernelGGL(member_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->i64c.p, L.j_off.p, t->mscr.p,
+                           L.j_idx.p);
+        SPHK_TRY(hipGetLastError());
+    }
     L.n_j = tot[0];
-    L.n_s = MODE == 2 ? tot[1] : 0;
-    L.built = true;
-    return SPHK_OK;
-}
-
-// Short-range lists in two stages: the split tree walk lists each walk's
-// candidate leaves (count, scan, fill), then member_kernel filters their
-// particles (count, scan, fill).  Same output as the one-stage walk.
-template <int MODE>
-int build_list_two_stage(sphk_dtree *t, const double *r)
-{
-    DLists &L = t->lists[MODE];
-    L.built = false;
-    const int ng = t->n_walk, nf = t->n_front;
-    const int64_t nt = (int64_t)ng * nf;
-    if (nt > std::numeric_limits<int>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many walk subtrees");
-    SPHK_TRY(t->i64a.reserve(nt, false, t->st));   // leaf counts per (walk, frontier node)
-    SPHK_TRY(t->i64c.reserve(nt, false, t->st));   // their scan
-    SPHK_TRY(t->total.reserve(2, false, t->st));
-    SPHK_TRY(t->leaf_off.reserve(ng + 1, false, t->st));
-    SPHK_TRY(L.j_off.reserve(ng + 1, false, t->st));
-    SPHK_TRY(t->wbox.reserve(6 * (size_t)ng, false, t->st));
-    hipLaunchKernelGGL(walk_box_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, t->i_off.p, t->x.p, t->y.p,
-                       t->z.p, r, t->wbox.p);
-    TreeView v{t->nodes(), t->moments(), t->brec.p, t->mrec.p, t->orec.p, t->x.p, t->y.p, t->z.p, t->group.p,
-               t->i_off.p};
-    const dim3 lg((unsigned)((nt + 63) / 64)), lb(64);
-    hipLaunchKernelGGL((list_kernel<MODE, false, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
-                       t->i64a.p, nullptr, nullptr, nullptr, nullptr, nullptr);
-    SPHK_TRY(hipGetLastError());
-    if (int rc = scan64(t, t->i64a.p, t->i64c.p, (int)nt)) return rc;
-    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, nf, t->i64c.p, t->i64a.p,
-                       t->leaf_off.p, t->total.p);
-    int64_t n_leaves = 0;
-    if (int rc = read1(t->st, t->total.p, &n_leaves)) return rc;
-    if (n_leaves > std::numeric_limits<int32_t>::max()) return set_error(SPHK_E_CAPACITY, "dtree: too many leaves");
-    SPHK_TRY(t->leaf_idx.reserve(std::max<int64_t>(n_leaves, 1), false, t->st));
-    hipLaunchKernelGGL((list_kernel<MODE, true, true>), lg, lb, 0, t->st, ng, nf, v, t->front.p, t->wbox.p, r, 0.,
-                       nullptr, nullptr, t->i64c.p, nullptr, t->leaf_idx.p, nullptr);
-    SPHK_TRY(hipGetLastError());
-    // stage 2: exact membership, one wave per walk, in one pass: the accepted
-    // particles go to the walk's stretch of a scratch array (room for all its
-    // candidates), then are copied to their offsets in the list
-    SPHK_TRY(t->i64b.reserve(ng, false, t->st));
-    SPHK_TRY(t->i64d.reserve(ng + 1, false, t->st));
-    hipLaunchKernelGGL(cand_total_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->leaf_off.p, t->leaf_idx.p,
-                       t->nodes(), t->i64b.p);
-    if (int rc = scan64(t, t->i64b.p, t->i64d.p, ng)) return rc;
-    int64_t n_cand = 0, last[2];
-    if (hipMemcpyAsync(&last[0], t->i64d.p + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, t->st) != hipSuccess ||
-        hipMemcpyAsync(&last[1], t->i64b.p + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, t->st) != hipSuccess ||
-        hipStreamSynchronize(t->st) != hipSuccess)
-        return set_error(SPHK_E_DEVICE, "dtree: candidate count read failed");
-    n_cand = last[0] + last[1];
-    SPHK_TRY(t->mscr.reserve(std::max<int64_t>(n_cand, 1), false, t->st));
-    hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, t->leaf_off.p, t->leaf_idx.p,
-                       t->wbox.p, r, t->i64b.p, t->i64d.p, t->mscr.p);
-    SPHK_TRY(hipGetLastError());
-    // i64b now holds the accepted counts; their scan gives the list offsets
-    SPHK_TRY(t->i64c.reserve(ng, false, t->st));
-    if (int rc = scan64(t, t->i64b.p, t->i64c.p, ng)) return rc;
-    hipLaunchKernelGGL(offsets_kernel, dim3(grid(ng)), dim3(kBlock), 0, t->st, ng, 1, t->i64c.p, t->i64b.p,
-                       L.j_off.p, t->total.p);
-    int64_t tot = 0;
-    if (int rc = read1(t->st, t->total.p, &tot)) return rc;
-    if (tot > std::numeric_limits<int32_t>::max()) return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
-    SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot, 1), false, t->st));
-    hipLaunchKernelGGL(member_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->i64d.p, L.j_off.p, t->mscr.p,
-                       L.j_idx.p);
-    SPHK_TRY(hipGetLastError());
-    L.n_j = tot;
-    L.n_s = 0;
+    L.n_s = tot[1];
     L.built = true;
     return SPHK_OK;
 }
@@ -1179,12 +1085,10 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
     t->st = stream;
     t->n_walk = 0;
     t->n_node = 0;
-    t->n_front = 0;
     for (auto &L : t->lists) L.built = false;
     auto fail = [&](int rc) {
         t->n_walk = 0;
         t->n_node = 0;
-        t->n_front = 0;
         return rc;
     };
     hipStream_t st = t->st;
@@ -1223,7 +1127,7 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
             return fail(set_error(SPHK_E_DEVICE, "dtree: level size read failed"));
         const int64_t nnew = last[0] + last[1];
         if (nnew == 0) break;
-        if (fb + nnew > std::numeric_limits<int32_t>::max())
+        if (fb + nnew >= (1 << 29))   // list walks pack node indices into 29 bits
             return fail(set_error(SPHK_E_CAPACITY, "dtree: too many nodes"));
         if ((size_t)(fb + nnew) > t->begin.cap)
             if (int rc = reserve_nodes(t, std::max<size_t>((size_t)(fb + nnew), t->begin.cap * 3 / 2))) return fail(rc);
@@ -1278,33 +1182,8 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
         return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
     hipLaunchKernelGGL(group_write_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64c.p, t->group.p,
                        t->i_off.p, ng);
-    if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
-        return fail(set_error(SPHK_E_DEVICE, "dtree: build failed"));
+    if (hipGetLastError() != hipSuccess) return fail(set_error(SPHK_E_DEVICE, "dtree: build failed"));
     t->n_walk = ng;
-
-    // frontier for the split list walks: the first level with >= 64 nodes
-    // (or the deepest), plus the leaves above it
-    int L = (int)t->level_start.size() - 2;
-    for (int q = 1; q + 1 < (int)t->level_start.size(); q++)
-        if (t->level_start[q + 1] - t->level_start[q] >= 64) {
-            L = q;
-            break;
-        }
-    hipLaunchKernelGGL(fill_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, -1);
-    hipLaunchKernelGGL(front_mark_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, st, (int)nn, t->nodes(), L,
-                       t->head.p);
-    hipLaunchKernelGGL(head_flag_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64a.p);
-    if (int rc = scan64(t, t->i64a.p, t->i64c.p, n)) return fail(rc);
-    if (hipMemcpyAsync(&last[0], t->i64c.p + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(&last[1], t->i64a.p + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return fail(set_error(SPHK_E_DEVICE, "dtree: frontier count read failed"));
-    t->n_front = (int)(last[0] + last[1]);
-    if (t->front.reserve(t->n_front, false, st) != hipSuccess)
-        return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
-    hipLaunchKernelGGL(front_write_kernel, dim3(grid(n)), dim3(kBlock), 0, st, n, t->head.p, t->i64c.p, t->front.p);
-    if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
-        return fail(set_error(SPHK_E_DEVICE, "dtree: frontier failed"));
     return SPHK_OK;
 }
 
@@ -1312,6 +1191,8 @@ int check_build_args(int n, const double *x, const double *y, const double *z, c
                      int n_group)
 {
     if (n <= 0 || !x || !y || !z || !m) return set_error(SPHK_E_ARG, "dtree: bad particle arrays");
+    // list walks pack node and particle indices into 29 bits
+    if (n >= (1 << 29)) return set_error(SPHK_E_CAPACITY, "dtree: at most 2^29 - 1 particles per tree");
     if (n_leaf < 1 || n_group < 1) return set_error(SPHK_E_ARG, "dtree: n_leaf, n_group must be >= 1");
     return SPHK_OK;
 }

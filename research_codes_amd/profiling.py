@@ -111,14 +111,13 @@ def counters(csv_path: str, kernel: str, last: int = 5) -> dict:
 
 _KDK_STAGES = (
     ("long list (EPJ + SPJ): count + fill", ("list_kernel<2",)),
-    ("symmetry list: candidate leaves + exact members", ("list_kernel<1", "member_kernel<1")),
-    ("gather list: candidate leaves + exact members", ("list_kernel<0", "member_kernel<0")),
-    ("list bookkeeping (candidate totals, copies, walk boxes, offsets)",
-     ("cand_total", "member_copy", "walk_box", "offsets_kernel")),
+    ("symmetry list: candidates count + fill, exact members", ("list_kernel<1", "member_kernel<1")),
+    ("gather list: candidates count + fill, exact members", ("list_kernel<0", "member_kernel<0")),
+    ("list bookkeeping (walk boxes, offsets, member copies)", ("walk_box", "offsets_kernel", "member_copy")),
     ("scans and sorts (rocPRIM)", ("rocprim",)),
-    ("tree rebuild (keys, levels, moments, node records, frontier)",
+    ("tree rebuild (keys, levels, moments, node records, groups)",
      ("split_kernel", "moments_kernel", "obox_kernel", "next_kernel", "pack_", "bbox_", "keys_kernel",
-      "unsorted_kernel", "root_kernel", "group_", "front_", "head_flag", "fill_kernel")),
+      "unsorted_kernel", "root_kernel", "group_", "head_flag", "fill_kernel")),
     ("density functor", ("density_kernel",)),
     ("gravity functor", ("gravity_kernel",)),
     ("hydro functor", ("hydro_kernel",)),
