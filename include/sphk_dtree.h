@@ -34,7 +34,9 @@ extern "C" {
 typedef struct sphk_dtree sphk_dtree;
 
 /* perm[k] = index of the particle that goes to sorted position k (ties in
- * key keep index order), as sphk_morton_order on the host. */
+ * key keep index order), as sphk_morton_order on the host.  Asynchronous:
+ * perm is written by work queued on `stream`.  A per-device workspace is kept
+ * between calls (grown as needed, held until the process ends). */
 int sphk_morton_order_dev(int n, const double *x, const double *y, const double *z, int32_t *perm,
                           void *stream);
 

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -604,13 +605,16 @@ __device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&
 // node to open, whose children the next step evaluates.  Emitted in this
 // order, the entries are the host's list element for element.  Each tree
 // level leaves at most 7 siblings pending, so the stack never holds more
-// than 7 (kMaxLevel + 1) + 8 entries.
+// than 7 (kMaxLevel + 1) + 8 entries (kListStack); walks run first with a
+// stack of kListStackSmall entries (less LDS, more waves in flight) and,
+// if one would overflow it, again with kListStack.
 //
 // Entry: word 0 the node (SPJ, and leaves in candidate-leaf mode), the first
 // child (to open) or the first particle (leaf, particles listed); word 1 the
 // type in bits 29-31 and the child or particle count below.
 constexpr int kListWalks = 8;
 constexpr int kListStack = 7 * (kMaxLevel + 1) + 8 + 2;   // 164 (even: 8-byte rows)
+constexpr int kListStackSmall = 64;
 constexpr uint32_t kTySpj = 1u << 29, kTyLeaf = 2u << 29, kTyOpen = 3u << 29, kTyMask = 7u << 29;
 constexpr uint32_t kCountMask = (1u << 29) - 1;
 
@@ -673,9 +677,9 @@ __device__ __forceinline__ bool member(const TreeView &t, const double *__restri
 // The particles of the leaves reached into jout (MODE 0 gather, 1 symmetry:
 // the candidates of the short-range lists), and for MODE 2 (long) the SPJ
 // nodes into sout: with FILL from the walk's offsets oj/os, else the walk's
-// counts go to cj/cs.  err is set if a stack would overflow (a corrupt
-// tree).
-template <int MODE, bool FILL>
+// counts go to cj/cs.  err is set if a stack of STACK entries would
+// overflow.
+template <int MODE, bool FILL, int STACK>
 __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const double *__restrict__ wb,
                                                        const double *__restrict__ r, double inv_t2,
                                                        int64_t *__restrict__ cj, int64_t *__restrict__ cs,
@@ -684,7 +688,7 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
                                                        int *__restrict__ err)
 {
 #pragma clang fp contract(off)
-    __shared__ uint2 stk[kListWalks][kListStack];
+    __shared__ uint2 stk[kListWalks][STACK];
     const int lane = threadIdx.x, sub = lane >> 3, c = lane & 7;
     const int g = blockIdx.x * kListWalks + sub;
     const bool live = g < ng;
@@ -754,7 +758,7 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
         emit(kept && c < fo, w0, w1);
         const bool push = kept && c >= fo;
         const unsigned pb = walk_bits(push, lane);
-        if (sp + __popc(pb) > kListStack) bad = true;
+        if (sp + __popc(pb) > STACK) bad = true;
         if (push && !bad) stk[sub][sp + __popc(pb & above)] = make_uint2(w0, w1);
         if (!bad) sp += __popc(pb);
         // pop finished entries down to the next node to open
@@ -935,14 +939,14 @@ int read1(hipStream_t st, const T *dev, T *host)
 
 int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
-// Both totals of a list: two int64 written by offsets_kernel (one host read).
-int read_totals(sphk_dtree *t, int64_t (&tot)[2])
+// Both totals of a list (two int64 written by offsets_kernel) and the list
+// walks' overflow flag, in one host read.
+int read_totals(sphk_dtree *t, int64_t (&tot)[2], int &overflow)
 {
-    int bad = 0;
+    overflow = 0;
     SPHK_TRY(hipMemcpyAsync(tot, t->total.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, t->st));
-    SPHK_TRY(hipMemcpyAsync(&bad, t->err.p, sizeof(int), hipMemcpyDeviceToHost, t->st));
+    SPHK_TRY(hipMemcpyAsync(&overflow, t->err.p, sizeof(int), hipMemcpyDeviceToHost, t->st));
     SPHK_TRY(hipStreamSynchronize(t->st));
-    if (bad) return set_error(SPHK_E_STATE, "dtree: list walk stack overflow (corrupt tree)");
     return SPHK_OK;
 }
 
@@ -990,34 +994,56 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
                t->i_off.p};
     const double *wb = MODE == 2 ? nullptr : t->wbox.p;
     const dim3 lg((unsigned)((ng + kListWalks - 1) / kListWalks)), lb(64);
-    hipLaunchKernelGGL((walk_list_kernel<MODE, false>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2, t->i64a.p,
-                       t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
-    SPHK_TRY(hipGetLastError());
-    if (int rc = walk_offsets(t, ng, t->i64a.p, t->i64c.p, L.j_off.p, MODE == 2 ? t->i64b.p : nullptr, t->i64d.p,
-                              L.s_off.p))
-        return rc;
+    // count walk with the small stack, and with the worst-case one if a walk overflowed it
     int64_t tot[2] = {0, 0};
-    if (int rc = read_totals(t, tot)) return rc;
+    int overflow = 0;
+    bool big = false;
+    for (;;) {
+        if (big)
+            hipLaunchKernelGGL((walk_list_kernel<MODE, false, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+                               t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
+        else
+            hipLaunchKernelGGL((walk_list_kernel<MODE, false, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
+                               inv_t2, t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
+        SPHK_TRY(hipGetLastError());
+        if (int rc = walk_offsets(t, ng, t->i64a.p, t->i64c.p, L.j_off.p, MODE == 2 ? t->i64b.p : nullptr,
+                                  t->i64d.p, L.s_off.p))
+            return rc;
+        if (int rc = read_totals(t, tot, overflow)) return rc;
+        if (!overflow) break;
+        if (big) return set_error(SPHK_E_STATE, "dtree: list walk stack overflow (corrupt tree)");
+        big = true;
+        SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
+    }
     if (MODE != 2) tot[1] = 0;
     if (tot[0] > std::numeric_limits<int32_t>::max() || tot[1] > std::numeric_limits<int32_t>::max())
         return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
     if constexpr (MODE == 2) {
         SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
         SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
-        hipLaunchKernelGGL((walk_list_kernel<MODE, true>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2, nullptr, nullptr,
-                           t->i64c.p, t->i64d.p, L.j_idx.p, L.s_idx.p, t->err.p);
+        if (big)
+            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+                               nullptr, nullptr, t->i64c.p, t->i64d.p, L.j_idx.p, L.s_idx.p, t->err.p);
+        else
+            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
+                               inv_t2, nullptr, nullptr, t->i64c.p, t->i64d.p, L.j_idx.p, L.s_idx.p, t->err.p);
         SPHK_TRY(hipGetLastError());
     } else {
         // tot[0] candidates into the scratch; the members compacted in place,
         // their counts to i64b
         SPHK_TRY(t->mscr.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
-        hipLaunchKernelGGL((walk_list_kernel<MODE, true>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2, nullptr, nullptr,
-                           t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
+        if (big)
+            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+                               nullptr, nullptr, t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
+        else
+            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
+                               inv_t2, nullptr, nullptr, t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
         hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, wb, r, t->i64c.p, t->i64a.p,
                            t->mscr.p, t->i64b.p);
         SPHK_TRY(hipGetLastError());
         if (int rc = walk_offsets(t, ng, t->i64b.p, t->i64d.p, L.j_off.p, nullptr, nullptr, nullptr)) return rc;
-        if (int rc = read_totals(t, tot)) return rc;
+        if (int rc = read_totals(t, tot, overflow)) return rc;
+        if (overflow) return set_error(SPHK_E_STATE, "dtree: list walk stack overflow in the fill walk");
         tot[1] = 0;
         if (tot[0] > std::numeric_limits<int32_t>::max())
             return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
@@ -1032,6 +1058,21 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
     return SPHK_OK;
 }
 
+// Workspace of sphk_morton_order_dev, one per device, grown as needed and
+// kept until the process ends (no allocation once it has reached the run's
+// size).  Calls on one device take it in turn; each call's stream first
+// waits for the previous call's work (an event), so no host wait is needed.
+struct MortonWork {
+    std::mutex mu;
+    DArr<double> part, frame;
+    DArr<uint64_t> key, key2;
+    DArr<int32_t> idx;
+    DArr<char> tmp;
+    hipEvent_t done = nullptr;
+};
+constexpr int kMortonDevices = 16;
+MortonWork g_morton[kMortonDevices];
+
 }  // namespace
 
 extern "C" {
@@ -1041,32 +1082,29 @@ int sphk_morton_order_dev(int n, const double *x, const double *y, const double 
     if (n < 0 || (n > 0 && (!x || !y || !z || !perm))) return set_error(SPHK_E_ARG, "morton_dev: bad args");
     if (n == 0) return SPHK_OK;
     hipStream_t st = (hipStream_t)stream;
-    DArr<double> part, frame;
-    DArr<uint64_t> key, key2;
-    DArr<int32_t> idx;
-    DArr<char> tmp;
-    int rc = SPHK_OK;
-    do {
-        if (idx.reserve(n, false, st) != hipSuccess || key2.reserve(n, false, st) != hipSuccess) {
-            rc = set_error(SPHK_E_DEVICE, "morton_dev: allocation failed");
-            break;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMortonDevices)
+        return set_error(SPHK_E_DEVICE, "morton_dev: no usable current device");
+    MortonWork &w = g_morton[dev];
+    std::lock_guard<std::mutex> lock(w.mu);
+    // the previous call's kernels (maybe on another stream) finish before the workspace is reused
+    if (!w.done) {
+        if (hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) {
+            w.done = nullptr;
+            return set_error(SPHK_E_DEVICE, "morton_dev: event creation failed");
         }
-        if ((rc = make_keys(n, x, y, z, st, part, frame, key, idx.p))) break;
-        size_t bytes = 0;
-        hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, key.p, key2.p, idx.p, perm, (size_t)n, 0, 63, st);
-        if (e == hipSuccess) e = tmp.reserve(bytes + 16, false, st);
-        if (e == hipSuccess)
-            e = rocprim::radix_sort_pairs(tmp.p, bytes, key.p, key2.p, idx.p, perm, (size_t)n, 0, 63, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = set_error(SPHK_E_DEVICE, "morton_dev: %s", hipGetErrorString(e));
-    } while (0);
-    part.release();
-    frame.release();
-    key.release();
-    key2.release();
-    idx.release();
-    tmp.release();
-    return rc;
+    } else {
+        SPHK_TRY(hipStreamWaitEvent(st, w.done, 0));
+    }
+    SPHK_TRY(w.idx.reserve(n, false, st));
+    SPHK_TRY(w.key2.reserve(n, false, st));
+    if (int rc = make_keys(n, x, y, z, st, w.part, w.frame, w.key, w.idx.p)) return rc;
+    size_t bytes = 0;
+    SPHK_TRY(rocprim::radix_sort_pairs(nullptr, bytes, w.key.p, w.key2.p, w.idx.p, perm, (size_t)n, 0, 63, st));
+    SPHK_TRY(w.tmp.reserve(bytes + 16, false, st));
+    SPHK_TRY(rocprim::radix_sort_pairs(w.tmp.p, bytes, w.key.p, w.key2.p, w.idx.p, perm, (size_t)n, 0, 63, st));
+    SPHK_TRY(hipEventRecord(w.done, st));
+    return SPHK_OK;
 }
 
 namespace {

@@ -76,6 +76,43 @@ def test_tree_and_lists_match_host(case, n_leaf, n_group):
         dt.close()
 
 
+def _fractal(levels=14):
+    """Particles whose octree opens 8 children at every one of `levels`
+    levels: 7 single-particle octants plus one that nests again; anchors at
+    the corners of the unit cube."""
+    pts = [(0.0, 0.0, 0.0), (1.0, 1.0, 1.0)]
+    for k in range(1, levels + 1):
+        h = 2.0 ** -k
+        for a in range(1, 8):
+            pts.append((((a >> 2) & 1) * h + 0.5 * h, ((a >> 1) & 1) * h + 0.5 * h, (a & 1) * h + 0.5 * h))
+    x, y, z = (np.array(v) for v in zip(*pts))
+    perm = morton_order(x, y, z)
+    n = len(x)
+    return {"x": x[perm], "y": y[perm], "z": z[perm], "m": np.full(n, 1.0 / n), "eps2": np.full(n, 1e-8),
+            "h": np.full(n, 3.0)}
+
+
+def test_deep_tree_lists_overflowing_the_small_stack():
+    """Every walk of this tree descends 14 levels with 7 siblings pending at
+    each (a preorder stack of about 100 entries): the device list walks
+    overflow their small LDS stack and run again with the worst-case one; the
+    lists must still equal the host's."""
+    p = _fractal()
+    ht = HostTree(p["x"], p["y"], p["z"], p["m"], p["eps2"], 1, 8)
+    dt = DeviceTree(_d(p["x"]), _d(p["y"]), _d(p["z"]), _d(p["m"]), _d(p["eps2"]), 1, 8)
+    try:
+        assert dt.n_node == ht.n_node and dt.n_walk == ht.n_walk
+        for name, hl, dl in (("gather", ht.gather(p["h"]), dt.gather(_d(p["h"]))),
+                             ("symmetry", ht.symmetry(p["h"]), dt.symmetry(_d(p["h"]))),
+                             ("long0", ht.long(0.0), dt.long(0.0))):
+            assert len(hl) == len(dl), name
+            for a, b in zip(hl, dl):
+                assert np.array_equal(a, b.cpu().numpy()), name
+    finally:
+        ht.close()
+        dt.close()
+
+
 def test_rebuild_in_place_matches_fresh_build():
     """One handle rebuilt for a sequence of particle sets (larger, smaller,
     another leaf and group size) gives exactly what a new handle gives."""
