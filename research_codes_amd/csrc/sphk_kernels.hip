@@ -105,6 +105,15 @@ __device__ __attribute__((noinline)) double exact_q(double dx, double dy, double
     return __builtin_sqrt(r2) * hinv;
 }
 
+// r = sqrt(r2) with the reference's FP64 rounding sequence: q = r * (1/H)
+// is then the reference's q bit for bit.
+__device__ __forceinline__ double exact_r(double dx, double dy, double dz)
+{
+#pragma clang fp contract(off)
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    return __builtin_sqrt(r2);
+}
+
 struct Pass1 {
     double rh;
     int nj;
@@ -248,6 +257,7 @@ __device__ __forceinline__ void dens_reduce_grad(double (&acc)[5], double *red, 
 // the iteration exceeds R_i the chunk continues with full sweeps.
 constexpr int kDensCap = 256;        // largest resident capacity (instantiated: 192, 256)
 constexpr int kDensComp = 1728;
+constexpr int kDensR = 12;           // members per lane held in registers for the h-passes
 static_assert(kDensCap <= 256, "compacted positions are stored as bytes");
 
 template <bool MIXED>
@@ -353,8 +363,10 @@ __device__ void density_streaming(int lane, const Map &M, int i, int jb, int je,
     if (M.lead) dens_finish(out, i, acc, dens, np, hs, I.rs);
 }
 
+// Launch bound: 3 waves per SIMD (at most 168 VGPRs; a few spill), the
+// occupancy the 192-record LDS layout allows.
 template <bool MIXED, int CAP>
-__global__ __launch_bounds__(kWave) void density_kernel(
+__global__ __launch_bounds__(kWave, 3) void density_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_dens_src pi, sphk_dens_src pj, sphk_dens_out out,
     double eta_ksrh, double hmax)
@@ -495,8 +507,74 @@ __global__ __launch_bounds__(kWave) void density_kernel(
         double dens = 0.;
         long long np = 0;
 
-        // -- three fixed-point passes: rho(H_k), n(H_k), H_{k+1} (hdr_density.hpp:264-302)
-        for (int rep = 0; rep < 3; rep++) {
+        // -- three fixed-point passes: rho(H_k), n(H_k), H_{k+1} (hdr_density.hpp:264-302).
+        // When every lane has at most kDensR members, their exact distances
+        // and masses are held in registers for the passes (one sqrt per
+        // member instead of one rsqrt refinement per member and pass); the
+        // passes that remain after an H beyond R_i run below.
+        int rep0 = 0;
+        if (nsteps <= kDensR) {
+            double rr[kDensR], mm[kDensR];
+#pragma unroll
+            for (int t = 0; t < kDensR; t++) {
+                if (t < nsteps) {
+                    const int e = e0 + t * stride;
+                    const double4 j = sp[e < n_e ? myidx[e] : kDensPad];
+                    rr[t] = exact_r(I.x - j.x, I.y - j.y, I.z - j.z);
+                    mm[t] = j.w;
+                }
+            }
+            for (; rep0 < 3; rep0++) {
+                const double hi = 1. / hs;
+                const double hi3 = hi * hi * hi;
+                double rh = 0.;
+                int nq = 0;
+#pragma unroll
+                for (int t = 0; t < kDensR; t++) {
+                    if (t < nsteps) {
+                        const double q = rr[t] * hi;
+                        rh += mm[t] * hi3 * kernel0th(q);
+                        nq += (q < 1.) ? 1 : 0;
+                    }
+                }
+                __syncthreads();
+                red[lane] = rh;
+                red[kWave + lane] = (double)nq;
+                __syncthreads();
+                double v0 = red[lane], v1 = red[kWave + lane];
+                if (owner)
+                    for (int t = 1; t < stride; t++) {
+                        v0 += red[lane + t];
+                        v1 += red[kWave + lane + t];
+                    }
+                int viol = 0;
+                if (owner) {
+                    dens = v0;
+                    np = (long long)v1;
+                    const double hnew = dens_next_h(eta_ksrh, I.m, dens, hmax);
+                    shs[my] = hnew;
+                    viol = hnew > (I.ksr > I.rs ? I.ksr : I.rs);
+                }
+                if (__syncthreads_or(viol)) {   // full sweeps from the next pass on
+                    if (owner) {
+                        red[my] = dens;
+                        red[kWave + my] = (double)np;
+                    }
+                    __syncthreads();
+                    compact = false;
+                    my = MF.il;
+                    owner = MF.lead;
+                    I = dens_load_i(pi, c0 + my);
+                    dens = red[my];
+                    np = (long long)red[kWave + my];
+                    hs = shs[my];
+                    rep0++;
+                    break;
+                }
+                hs = shs[my];
+            }
+        }
+        for (int rep = rep0; rep < 3; rep++) {
             const double hi = 1. / hs;
             const double hi3 = hi * hi * hi;
             Pass1 a{0., 0};

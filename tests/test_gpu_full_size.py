@@ -99,3 +99,26 @@ def test_full_size_mixed_matches_fp64(system):
     rel = ((a_got - a_ref).norm(dim=0) / a_ref.norm(dim=0).clamp_min(1e-300))
     print(f"[full] mixed vs FP64 acceleration: max {float(rel.max()):.2e}, median {float(rel.median()):.2e}")
     assert float(rel.max()) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_full_size_sampled_walks_match_oracle(system, prec):
+    """At full size, with viscosity and conduction live (rotating, noisy
+    stars, alpha_u = 0.5): the three functors on 16 evenly spaced blocks of
+    8 walks against the oracle, np exact, FP64 to 1e-10 and mixed to 1e-4 of
+    each field's scale; div v, curl v, du/dt and the conduction term are
+    nonzero in the sampled walks."""
+    from tests._cases import check_walks_against_oracle
+    s, snap = system
+    for k, v in snap.items():
+        s.s[k].copy_(v)
+    s.force(FP64)            # EOS, softening, eta and the hydro inputs of this state
+    torch.cuda.synchronize()
+    errs, np_mis = check_walks_against_oracle(s, prec, n_blocks=16, per_block=8)
+    worst = max(errs, key=errs.get)
+    print(f"[full] oracle on sampled walks ({prec}): worst {worst} {errs[worst]:.2e}, np mismatches {np_mis}")
+    assert np_mis == 0
+    assert errs[worst] < (1e-10 if prec == FP64 else 1e-4)
+    for k in ("dens.divv", "dens.rotv", "hydro.udot", "hydro.diffu"):
+        assert k in errs
+    assert float(s.s["divv"].abs().max()) > 0 and float(s.hout["udot"].abs().max()) > 0
