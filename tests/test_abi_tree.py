@@ -76,13 +76,20 @@ def _walk_of(i_off, n):
 
 
 @pytest.mark.parametrize("n_group", [8, 16, 64, 256])
-def test_gather_and_symmetry_lists_are_supersets(star, n_group):
+def test_gather_and_symmetry_lists_are_exact_spheres(star, n_group):
+    """Each walk's gather list is exactly the particles within rs_i of one of
+    its particles, the symmetry list exactly those within max(H_i, H_j), by
+    brute force over all pairs with the builder's squared distance
+    ((dx dx + dy dy) + dz dz, no contraction) against the squared radius;
+    no duplicates."""
     p = star
     n = len(p["m"])
     t = HostTree(p["x"], p["y"], p["z"], p["m"], None, 8, n_group)
     assert np.diff(t.i_off).max() <= max(n_group, 8)
-    X = np.stack([p["x"], p["y"], p["z"]], 1)
-    r = np.sqrt(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1))
+    dx = p["x"][:, None] - p["x"][None, :]
+    dy = p["y"][:, None] - p["y"][None, :]
+    dz = p["z"][:, None] - p["z"][None, :]
+    d2 = (dx * dx + dy * dy) + dz * dz
     rs = 1.1 * p["ksr"]
     jo, ji = t.gather(rs)
     h = p["ksr"]
@@ -90,12 +97,13 @@ def test_gather_and_symmetry_lists_are_supersets(star, n_group):
     for w in range(t.n_walk):
         ii = np.arange(t.i_off[w], t.i_off[w + 1])
         glist = set(ji[jo[w]:jo[w + 1]].tolist())
-        need = set(np.nonzero((r[ii] < rs[ii, None]).any(0))[0].tolist())
-        assert need <= glist
+        need = set(np.nonzero((d2[ii] < (rs[ii] * rs[ii])[:, None]).any(0))[0].tolist())
+        assert need == glist
         slist = set(si[so[w]:so[w + 1]].tolist())
-        need_s = set(np.nonzero((r[ii] < np.maximum(h[ii, None], h[None, :])).any(0))[0].tolist())
-        assert need_s <= slist
-        assert len(glist) == jo[w + 1] - jo[w]          # no duplicates
+        hm = np.maximum(h[ii, None], h[None, :])
+        need_s = set(np.nonzero((d2[ii] < hm * hm).any(0))[0].tolist())
+        assert need_s == slist
+        assert len(glist) == jo[w + 1] - jo[w] and len(slist) == so[w + 1] - so[w]   # no duplicates
     t.close()
 
 
