@@ -152,7 +152,9 @@ int sphk_hydro_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_
 /* Gravity functor over the EPJ list then the SPJ list of each walk; replaces
  * GravityForceKernel (hdr_gravityGPU.cu:330-374) launched by
  * DispatchKernel_Gravity (:383-469).  s_off may be NULL (no SPJ).  Outputs
- * are assigned (as the reference's GPU Retrieve does, :485-489). */
+ * are assigned (as the reference's GPU Retrieve does, :485-489).  out.eta
+ * may be NULL: the softening-gradient sum is then not formed (callers that
+ * do not use it, e.g. under a constant softening). */
 int sphk_gravity_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off,
                      const int32_t *j_off, const int32_t *j_idx, sphk_grav_src pj,
                      const int32_t *s_off, const int32_t *s_idx, sphk_grav_src ps,

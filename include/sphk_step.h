@@ -72,7 +72,7 @@ int sphk_softening_dev(sphk_ctx *ctx, int n, const double *ksr, double ksrh, dou
                        double *eps2, void *stream);
 
 /* accg2, pot and eta from the gravity functor's outputs (eta = 0 when
- * p.eps_const > 0). */
+ * p.eps_const > 0; the functor's eta output may then be NULL). */
 int sphk_gravity_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p,
                                sphk_grav_out g, void *stream);
 

@@ -75,6 +75,8 @@ __device__ __forceinline__ double cbrt_f64(double x)
 }
 
 __device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ double rsqrt_r(double x) { return rsqrt_f64(x); }
 __device__ __forceinline__ float rcp_r(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float abs_r(float x) { return __builtin_fabsf(x); }

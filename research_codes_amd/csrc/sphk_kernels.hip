@@ -973,7 +973,7 @@ struct GravAcc {
 // General pair term (softening of i and of j may differ):
 // acc -= m x (rei3_i + rei3_j), pot -= m (rei_i + rei_j), eta += m rei3_i;
 // the per-particle totals are halved at the end (hdr_gravity.hpp:310-331).
-template <bool MIXED>
+template <bool MIXED, bool ETA>
 __device__ __forceinline__ void gravity_pair(const GRec<MIXED> &p, double px, double py, double pz,
                                              real_t<MIXED> e2i, GravAcc<real_t<MIXED>> &a)
 {
@@ -992,13 +992,13 @@ __device__ __forceinline__ void gravity_pair(const GRec<MIXED> &p, double px, do
     a.ax -= dpx * dg2;
     a.ay -= dpy * dg2;
     a.az -= dpz * dg2;
-    a.et += m_j * rei3_i;
+    if (ETA) a.et += m_j * rei3_i;
 }
 
 // Uniform softening (every eps2 equal, e.g. a constant gravitational
 // softening): the two symmetrised terms coincide, one rsqrt per pair.
 // Accumulates +m x rei3, +m rei, +m rei3; the caller finishes with the sign.
-template <bool MIXED>
+template <bool MIXED, bool ETA>
 __device__ __forceinline__ void gravity_pair_uniform(const GRec<MIXED> &p, double px, double py,
                                                      double pz, real_t<MIXED> e2,
                                                      GravAcc<real_t<MIXED>> &a)
@@ -1007,8 +1007,9 @@ __device__ __forceinline__ void gravity_pair_uniform(const GRec<MIXED> &p, doubl
     const R dpx = (R)(px - p.x);
     const R dpy = (R)(py - p.y);
     const R dpz = (R)(pz - p.z);
-    const R r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-    const R rei = rsqrt_r(r2 + e2);
+    // r^2 + eps^2 with the softening in the first FMA
+    const R r2e = fma_r(dpz, dpz, fma_r(dpy, dpy, fma_r(dpx, dpx, e2)));
+    const R rei = rsqrt_r(r2e);
     const R m_j = (R)p.m;
     const R mr = m_j * rei;
     const R w = mr * rei * rei;
@@ -1016,10 +1017,10 @@ __device__ __forceinline__ void gravity_pair_uniform(const GRec<MIXED> &p, doubl
     a.ax += dpx * w;
     a.ay += dpy * w;
     a.az += dpz * w;
-    a.et += w;
+    if (ETA) a.et += w;
 }
 
-template <bool MIXED, bool UNIFORM>
+template <bool MIXED, bool UNIFORM, bool ETA>
 __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int ke,
                                              const int32_t *__restrict__ idx, sphk_grav_src src,
                                              double px, double py, double pz, double e2_i,
@@ -1042,22 +1043,22 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
         for (; k + 3 * M.S < nt; k += 4 * M.S) {
             const GRec<MIXED> p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
             if (UNIFORM) {
-                gravity_pair_uniform<MIXED>(p0, px, py, pz, e2, a);
-                gravity_pair_uniform<MIXED>(p1, px, py, pz, e2, a);
-                gravity_pair_uniform<MIXED>(p2, px, py, pz, e2, a);
-                gravity_pair_uniform<MIXED>(p3, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED, ETA>(p0, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED, ETA>(p1, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED, ETA>(p2, px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED, ETA>(p3, px, py, pz, e2, a);
             } else {
-                gravity_pair<MIXED>(p0, px, py, pz, e2, a);
-                gravity_pair<MIXED>(p1, px, py, pz, e2, a);
-                gravity_pair<MIXED>(p2, px, py, pz, e2, a);
-                gravity_pair<MIXED>(p3, px, py, pz, e2, a);
+                gravity_pair<MIXED, ETA>(p0, px, py, pz, e2, a);
+                gravity_pair<MIXED, ETA>(p1, px, py, pz, e2, a);
+                gravity_pair<MIXED, ETA>(p2, px, py, pz, e2, a);
+                gravity_pair<MIXED, ETA>(p3, px, py, pz, e2, a);
             }
         }
         for (; k < nt; k += M.S) {
             if (UNIFORM)
-                gravity_pair_uniform<MIXED>(sp[k], px, py, pz, e2, a);
+                gravity_pair_uniform<MIXED, ETA>(sp[k], px, py, pz, e2, a);
             else
-                gravity_pair<MIXED>(sp[k], px, py, pz, e2, a);
+                gravity_pair<MIXED, ETA>(sp[k], px, py, pz, e2, a);
         }
         acc[0] += (double)a.ax;
         acc[1] += (double)a.ay;
@@ -1072,23 +1073,25 @@ __device__ __forceinline__ void gravity_list(int lane, const Map &M, int kb, int
 // formed in FP64, so the pair term needs no FP64 arithmetic.  The offset of
 // a far record carries a relative rounding error of 2^-24 of its distance;
 // for a near pair the error is 2^-24 of its distance from the reference point.
+template <bool ETA>
 __device__ __forceinline__ void gravity_pair_rel(const float4 &p, float xi, float yi, float zi, float e2,
                                                  GravAcc<float> &a)
 {
     const float dx = xi - p.x;
     const float dy = yi - p.y;
     const float dz = zi - p.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const float rei = rsqrt_r(r2 + e2);
+    // r^2 + eps^2 with the softening in the first FMA
+    const float rei = rsqrt_r(fma_r(dz, dz, fma_r(dy, dy, fma_r(dx, dx, e2))));
     const float mr = p.w * rei;
     const float w = mr * rei * rei;
     a.pt += mr;
     a.ax += dx * w;
     a.ay += dy * w;
     a.az += dz * w;
-    a.et += w;
+    if (ETA) a.et += w;
 }
 
+template <bool ETA>
 __device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb, int ke,
                                                  const int32_t *__restrict__ idx, const sphk_grav_src &src,
                                                  double cx, double cy, double cz, float xi, float yi,
@@ -1109,12 +1112,12 @@ __device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb,
         int k = M.slice;
         for (; k + 3 * M.S < nt; k += 4 * M.S) {
             const float4 p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
-            gravity_pair_rel(p0, xi, yi, zi, e2, a);
-            gravity_pair_rel(p1, xi, yi, zi, e2, a);
-            gravity_pair_rel(p2, xi, yi, zi, e2, a);
-            gravity_pair_rel(p3, xi, yi, zi, e2, a);
+            gravity_pair_rel<ETA>(p0, xi, yi, zi, e2, a);
+            gravity_pair_rel<ETA>(p1, xi, yi, zi, e2, a);
+            gravity_pair_rel<ETA>(p2, xi, yi, zi, e2, a);
+            gravity_pair_rel<ETA>(p3, xi, yi, zi, e2, a);
         }
-        for (; k < nt; k += M.S) gravity_pair_rel(sp[k], xi, yi, zi, e2, a);
+        for (; k < nt; k += M.S) gravity_pair_rel<ETA>(sp[k], xi, yi, zi, e2, a);
         acc[0] += (double)a.ax;
         acc[1] += (double)a.ay;
         acc[2] += (double)a.az;
@@ -1123,7 +1126,9 @@ __device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb,
     }
 }
 
-template <bool MIXED, bool UNIFORM>
+// ETA false: the caller does not need the softening-gradient sum (out.eta
+// NULL, e.g. under a constant softening, where it does not enter).
+template <bool MIXED, bool UNIFORM, bool ETA>
 __global__ __launch_bounds__(kWave) void gravity_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_grav_src pj, const int32_t *__restrict__ s_off,
@@ -1149,17 +1154,17 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
             const double cx = pi.x[ib], cy = pi.y[ib], cz = pi.z[ib];
             const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
             float4 *sr = reinterpret_cast<float4 *>(smem);
-            gravity_list_rel(M, lane, j_off[w], j_off[w + 1], j_idx, pj, cx, cy, cz, xi, yi, zi, (float)e2,
-                             sr, acc);
+            gravity_list_rel<ETA>(M, lane, j_off[w], j_off[w + 1], j_idx, pj, cx, cy, cz, xi, yi, zi,
+                                  (float)e2, sr, acc);
             if (s_off)
-                gravity_list_rel(M, lane, s_off[w], s_off[w + 1], s_idx, ps, cx, cy, cz, xi, yi, zi,
-                                 (float)e2, sr, acc);
+                gravity_list_rel<ETA>(M, lane, s_off[w], s_off[w + 1], s_idx, ps, cx, cy, cz, xi, yi, zi,
+                                      (float)e2, sr, acc);
         } else {
-            gravity_list<MIXED, UNIFORM>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
-                                         acc);
+            gravity_list<MIXED, UNIFORM, ETA>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
+                                              acc);
             if (s_off)
-                gravity_list<MIXED, UNIFORM>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
-                                             sp, acc);
+                gravity_list<MIXED, UNIFORM, ETA>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
+                                                  sp, acc);
         }
         slice_reduce<5>(acc, red, M, lane);
         if (M.lead) {
@@ -1175,7 +1180,7 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
                 out.az[i] = acc[2] * 0.5;
                 out.pot[i] = acc[3] * 0.5;
             }
-            out.eta[i] = acc[4];
+            if (ETA) out.eta[i] = acc[4];
         }
     }
 }
@@ -1226,13 +1231,23 @@ hipError_t launch_gravity(hipStream_t st, int n_walk, const int32_t *i_off, cons
                           const sphk_grav_out &out, bool mixed, bool uniform)
 {
     if (n_walk <= 0) return hipSuccess;
-#define SPHK_LAUNCH_G(M, U)                                                                         \
-    hipLaunchKernelGGL((gravity_kernel<M, U>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, \
+#define SPHK_LAUNCH_G(M, U, E)                                                                         \
+    hipLaunchKernelGGL((gravity_kernel<M, U, E>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, \
                        j_idx, pj, s_off, s_idx, ps, pi, out)
-    if (mixed && uniform) SPHK_LAUNCH_G(true, true);
-    else if (mixed) SPHK_LAUNCH_G(true, false);
-    else if (uniform) SPHK_LAUNCH_G(false, true);
-    else SPHK_LAUNCH_G(false, false);
+    const bool eta = out.eta != nullptr;
+    if (mixed && uniform) {
+        if (eta) SPHK_LAUNCH_G(true, true, true);
+        else SPHK_LAUNCH_G(true, true, false);
+    } else if (mixed) {
+        if (eta) SPHK_LAUNCH_G(true, false, true);
+        else SPHK_LAUNCH_G(true, false, false);
+    } else if (uniform) {
+        if (eta) SPHK_LAUNCH_G(false, true, true);
+        else SPHK_LAUNCH_G(false, true, false);
+    } else {
+        if (eta) SPHK_LAUNCH_G(false, false, true);
+        else SPHK_LAUNCH_G(false, false, false);
+    }
 #undef SPHK_LAUNCH_G
     return hipGetLastError();
 }

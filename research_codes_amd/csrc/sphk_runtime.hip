@@ -466,7 +466,7 @@ int sphk_gravity_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
         return set_error(SPHK_E_ARG, "gravity: null field pointer");
     if (s_off && (!ps.m || !ps.x || !ps.y || !ps.z || !ps.eps2))
         return set_error(SPHK_E_ARG, "gravity: null SPJ field pointer");
-    if (!out.ax || !out.ay || !out.az || !out.pot || !out.eta)
+    if (!out.ax || !out.ay || !out.az || !out.pot)   // out.eta may be NULL: not formed
         return set_error(SPHK_E_ARG, "gravity: null output pointer");
     hipStream_t st = pick_stream(ctx, stream);
     Timer t(ctx, 2, st);

@@ -57,7 +57,10 @@ __global__ __launch_bounds__(kBlock) void gravity_writeback_kernel(int n, sphk_s
     s.g2z[i] = p.G * g.az[i];
     // hdr_gravity.hpp:110-111: eta is dPhi/dH through eps = H/kappa; with a
     // constant softening the potential does not depend on H and eta is 0
-    s.eta[i] = p.eps_const > 0. ? 0. : h * h * kinv * kinv * s.grdh[i] / (3. * s.dens[i]) * g.eta[i];
+    if (p.eps_const > 0.)
+        s.eta[i] = 0.;
+    else
+        s.eta[i] = h * h * kinv * kinv * s.grdh[i] / (3. * s.dens[i]) * g.eta[i];
     s.pot[i] = p.G * g.pot[i];
 }
 
@@ -406,7 +409,8 @@ int sphk_gravity_writeback_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, 
 {
     int rc = pre(ctx, n, &s);
     if (rc || n == 0) return rc;
-    if (!g.ax || !g.ay || !g.az || !g.pot || !g.eta) return set_error(SPHK_E_ARG, "gravity_writeback: null");
+    if (!g.ax || !g.ay || !g.az || !g.pot || (!g.eta && !(p.eps_const > 0.)))
+        return set_error(SPHK_E_ARG, "gravity_writeback: null (eta may be NULL only under a constant softening)");
     hipLaunchKernelGGL(gravity_writeback_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n,
                        s, p, g);
     return launch_check("gravity_writeback");

@@ -397,9 +397,11 @@ class SPHSystem:
         self._softening()
         s = self.s
         src = dict(m=s["m"], x=s["x"], y=s["y"], z=s["z"], eps2=self.eps2)
-        self.ctx.gravity(self.lists["grav"], src, src, self.gout, ps=self.mom,
+        # under a constant softening the softening-gradient sum does not enter
+        gout = self.gout if self.eps is None else {k: v for k, v in self.gout.items() if k != "eta"}
+        self.ctx.gravity(self.lists["grav"], src, src, gout, ps=self.mom,
                          precision=self.precision if precision is None else precision)
-        go = L.GravOut(**{k: self.gout[k].data_ptr() for k in L.GRAV_OUT})
+        go = L.GravOut(**{k: gout[k].data_ptr() if k in gout else None for k in L.GRAV_OUT})
         check(lib().sphk_gravity_writeback_dev(self.ctx.handle, self.n, self._ss, self.phys, go,
                                                self._st()), "sphk_gravity_writeback_dev")
 

@@ -129,6 +129,16 @@ def test_gravity_uniform_softening(ctx, prec):
         e = scaled_err(out[k].cpu().numpy(), ref[k], floor_frac=1e-1 if k in ("ax", "ay", "az") else 1e-3)
         print(f"gravity-uniform[{prec}] {k}: {e:.3e}")
         assert e < tol, k
+    # without the eta output (not formed): the other outputs bit for bit
+    out2 = {k: torch.full_like(v, 7.0) for k, v in out.items() if k != "eta"}
+    ctx.set_params(uniform_softening=True)
+    try:
+        ctx.gravity(w, src, src, out2, ps=mom, precision=prec)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_params(uniform_softening=False)
+    for k, v in out2.items():
+        assert torch.equal(v, out[k]), k
 
 
 def _density_vs_oracle(ctx, p, i_off, j_off, j_idx, eta, hmax, prec, label):
