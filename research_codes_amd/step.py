@@ -398,7 +398,7 @@ class SPHSystem:
         s = self.s
         src = dict(m=s["m"], x=s["x"], y=s["y"], z=s["z"], eps2=self.eps2)
         # under a constant softening the softening-gradient sum does not enter
-        gout = self.gout if self.eps is None else {k: v for k, v in self.gout.items() if k != "eta"}
+        gout = {k: v for k, v in self.gout.items() if k != "eta"} if self.phys.eps_const > 0 else self.gout
         self.ctx.gravity(self.lists["grav"], src, src, gout, ps=self.mom,
                          precision=self.precision if precision is None else precision)
         go = L.GravOut(**{k: gout[k].data_ptr() if k in gout else None for k in L.GRAV_OUT})
