@@ -52,6 +52,17 @@ def list_entries(w) -> int:
     return int(w.entries + w.entries_spj)
 
 
+def streamed_share(w, cap: int) -> dict:
+    """Share of the density walks (and of their list entries) whose list is
+    longer than the kernel's LDS-resident capacity and is streamed instead."""
+    n_i = (w.i_off[1:] - w.i_off[:-1]).double()
+    n_j = (w.j_off[1:] - w.j_off[:-1]).double()
+    long = n_j > cap
+    ent = n_i * n_j
+    return {"walks": round(float(long.double().mean()), 4),
+            "entries": round(float(ent[long].sum() / ent.sum()), 4)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +257,7 @@ def main() -> int:
         "autotune_ms": None if tune is None else {k: {str(g): round(t, 4) for g, t in v.items()}
                                                   for k, v in tune[1].items()},
         "dens_list_cap": sysm.ctx.params["list_cap"] or 256,
+        "dens_streamed": streamed_share(W["dens"], sysm.ctx.params["list_cap"] or 256),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dropin_fdps_path": dropin,
