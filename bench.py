@@ -515,23 +515,24 @@ def _host_inputs(sysm):
     return dsrc, hsrc, gsrc, mom, lists
 
 
-def _oracle_step(sysm, nthreads, inputs=None):
+def _oracle_step(sysm, nthreads, inputs=None, simd=False):
     """The oracle's three functors over all walks of the system's lists;
-    returns (outputs per functor, seconds per functor)."""
+    returns (outputs per functor, seconds per functor).  simd: the vector
+    functors (four i-particles per AVX2 vector, refined reciprocals)."""
     import oracle
     dsrc, hsrc, gsrc, mom, L = inputs or _host_inputs(sysm)
     out, sec = {}, {}
     t = time.perf_counter()
     out["dens"] = oracle.density(*L["dens"][:3], dsrc, dsrc, eta=sysm.eta, hmax=sysm.ctx.params["hmax"],
-                                 nthreads=nthreads)
+                                 nthreads=nthreads, avx2=simd, simd=simd)
     sec["density"] = time.perf_counter() - t
     t = time.perf_counter()
-    out["hydro"] = oracle.hydro(*L["hydro"][:3], hsrc, hsrc, nthreads=nthreads)
+    out["hydro"] = oracle.hydro(*L["hydro"][:3], hsrc, hsrc, nthreads=nthreads, simd=simd)
     sec["hydro"] = time.perf_counter() - t
     i_off, j_off, j_idx, s_off, s_idx = L["grav"]
     t = time.perf_counter()
     out["grav"] = oracle.gravity(i_off, j_off, j_idx, gsrc, gsrc, s_off=s_off, s_idx=s_idx, ps=mom,
-                                 nthreads=nthreads)
+                                 nthreads=nthreads, simd=simd)
     sec["gravity"] = time.perf_counter() - t
     return out, sec
 
@@ -573,17 +574,22 @@ def _parity(gpu, ref):
 
 
 def cpu_baseline(sysm, entries, log):
-    """The oracle (scalar FP64 C, OpenMP over walks on up to 16 host cores)
-    running the same force step -- the three functors over every walk of the
-    same lists and inputs -- timed whole, beside the CPU-runnable
-    configuration (N = 1e4, one step) timed whole; the GPU functors' outputs
-    (FP64 and mixed) are checked against the oracle's for every particle."""
+    """The same force step on the host (OpenMP over walks, up to 16 cores) --
+    the three functors over every walk of the same lists and inputs -- timed
+    whole in the reference's shipped CPU form (its USE_INTRINSICS build: four
+    i-particles per AVX2 vector, refined reciprocals; the oracle's vector
+    functors) and in the scalar oracle's.  The scalar oracle's outputs check
+    the GPU functors (FP64 and mixed) on every particle."""
     from research_codes_amd.device import FP64, MIXED
     nthreads = min(16, os.cpu_count() or 1)
-    ref, sec = _oracle_step(sysm, nthreads)
+    inputs = _host_inputs(sysm)
+    ref, sec_scalar = _oracle_step(sysm, nthreads, inputs)
+    vec, sec = _oracle_step(sysm, nthreads, inputs, simd=True)
     step_s = sum(sec.values())
     total = sum(entries.values())
-    log("cpu baseline (oracle, whole step): " + ", ".join(f"{k} {v:.2f}s" for k, v in sec.items()))
+    log("cpu baseline (vector functors, whole step): " + ", ".join(f"{k} {v:.2f}s" for k, v in sec.items())
+        + "; scalar oracle: " + ", ".join(f"{k} {v:.2f}s" for k, v in sec_scalar.items()))
+    vec_err = _parity(vec, ref)
     parity = {"particles": sysm.n}
     for name, prec in (("fp64", FP64), ("mixed", MIXED)):
         parity[name] = _parity(_gpu_outputs(sysm, prec), ref)
@@ -593,10 +599,14 @@ def cpu_baseline(sysm, entries, log):
     return {"value": total / step_s, "unit": "interactions/s", "cores": nthreads, "kind": "port",
             "cpu": cpu_model(), "ms_per_step": round(step_s * 1e3, 1),
             "seconds_per_functor": {k: round(v, 3) for k, v in sec.items()},
-            "arithmetic": "scalar FP64 (the reference's shipped CPU path is its AVX2 branch, whose refined "
-                          "reciprocals differ from the scalar functor by < 1e-12: tests/test_oracle_avx2.py)",
-            "sample": "the whole force step: the oracle's density, hydro and gravity functors over all walks "
-                      "of the GPU step's lists and inputs, once, OpenMP over walks",
+            "arithmetic": "FP64, four i-particles per AVX2 vector with the refined reciprocals rcp_4th / "
+                          "rsqrt_4th: the reference's shipped USE_INTRINSICS functors (oracle/sphk_oracle_simd.c; "
+                          "tests/test_oracle_avx2.py)",
+            "vector_vs_scalar_oracle": {"max_rel_err": vec_err["max_rel_err"], "np_mismatch": vec_err["np_mismatch"]},
+            "scalar_oracle": {"ms_per_step": round(sum(sec_scalar.values()) * 1e3, 1),
+                              "seconds_per_functor": {k: round(v, 3) for k, v in sec_scalar.items()}},
+            "sample": "the whole force step: density, hydro and gravity over all walks of the GPU step's lists "
+                      "and inputs, once, OpenMP over walks",
             "parity_vs_gpu": parity}
 
 
@@ -604,7 +614,8 @@ def cfg1_step(log):
     """The CPU-runnable configuration whole: one density + hydro + gravity
     force step of an N = 1e4 n=1.5 star (eta = 1.25, theta = 0.5, constant
     softening 0.01) on the GPU (mixed and FP64, HIP events around the step)
-    and by the oracle on the same lists and inputs (host cores)."""
+    and on the host (the vector functors and the scalar oracle) on the same
+    lists and inputs."""
     import torch
     from research_codes_amd import ic
     from research_codes_amd.device import FP64, MIXED, Context
@@ -633,12 +644,15 @@ def cfg1_step(log):
             res[f"gpu_ms_{name}"] = round(float(np.median(ms[1:])), 4)
         s._sbuf.copy_(snap)
         nthreads = min(16, os.cpu_count() or 1)
-        _, sec = _oracle_step(s, nthreads)
+        inputs = _host_inputs(s)
+        _, sec = _oracle_step(s, nthreads, inputs, simd=True)
+        res["cpu_vector_ms"] = round(sum(sec.values()) * 1e3, 2)
+        _, sec = _oracle_step(s, nthreads, inputs)
         res["oracle_ms"] = round(sum(sec.values()) * 1e3, 2)
         res["oracle_cores"] = nthreads
         res["mean_np"] = round(float(s.np_.double().mean()), 2)
     log(f"N=1e4 step: GPU {res['gpu_ms_mixed']} ms mixed, {res['gpu_ms_fp64']} ms FP64; "
-        f"oracle {res['oracle_ms']} ms")
+        f"CPU vector functors {res['cpu_vector_ms']} ms, scalar oracle {res['oracle_ms']} ms")
     return res
 
 

@@ -170,9 +170,10 @@ def _walk_args(i_off, j_off, j_idx):
 
 
 def density(i_off, j_off, j_idx, pi: dict, pj: dict, eta=1.6, ksrh=1.936492, hmax=1e30,
-            nthreads=0, avx2=False) -> dict:
+            nthreads=0, avx2=False, simd=False) -> dict:
     """Scalar calcDensity over walks.  pi/pj: dicts of SoA arrays.  avx2=True:
-    the AVX2 branch's arithmetic (approximate reciprocals) instead."""
+    the AVX2 branch's arithmetic (approximate reciprocals) instead; simd=True:
+    the same arithmetic four i-particles per AVX2 vector (bitwise equal)."""
     i_off, j_off, j_idx = _walk_args(i_off, j_off, j_idx)
     keep = []
 
@@ -187,14 +188,25 @@ def density(i_off, j_off, j_idx, pi: dict, pj: dict, eta=1.6, ksrh=1.936492, hma
     out["np"] = np.zeros(ni, np.int64)
     out["itr"] = np.zeros(ni, np.uint8)
     o = DensOut(**{k: _ptr(v) for k, v in out.items()})
-    fn = lib().orc_density_avx2 if avx2 else lib().orc_density
+    if simd:
+        fn = _simd().orc_density_simd
+    else:
+        fn = lib().orc_density_avx2 if avx2 else lib().orc_density
     fn(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pi, True),
                       src(pj, False), C.c_double(eta), C.c_double(ksrh), C.c_double(hmax), o,
                       int(nthreads))
     return out
 
 
-def hydro(i_off, j_off, j_idx, pi: dict, pj: dict, nthreads=0) -> dict:
+def _simd():
+    if not lib().orc_density_simd_supported():
+        raise RuntimeError("the vector functors need a CPU with AVX2")
+    return lib()
+
+
+def hydro(i_off, j_off, j_idx, pi: dict, pj: dict, nthreads=0, simd=False) -> dict:
+    """Scalar calcHydro over walks; simd=True: four i-particles per AVX2
+    vector with the refined reciprocals (the reference's vector build)."""
     i_off, j_off, j_idx = _walk_args(i_off, j_off, j_idx)
     keep = []
 
@@ -206,13 +218,15 @@ def hydro(i_off, j_off, j_idx, pi: dict, pj: dict, nthreads=0) -> dict:
     ni = int(i_off[-1])
     out = {k: np.zeros(ni) for k in ("ax", "ay", "az", "gx", "gy", "gz", "udot", "vsmx", "diffu")}
     o = HydroOut(**{k: _ptr(v) for k, v in out.items()})
-    lib().orc_hydro(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pi), src(pj), o,
-                    int(nthreads))
+    fn = _simd().orc_hydro_simd if simd else lib().orc_hydro
+    fn(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pi), src(pj), o, int(nthreads))
     return out
 
 
 def gravity(i_off, j_off, j_idx, pj: dict, pi: dict, s_off=None, s_idx=None, ps: dict | None = None,
-            nthreads=0) -> dict:
+            nthreads=0, simd=False) -> dict:
+    """Scalar calcGravity over walks (EPJ then SPJ list); simd=True: four
+    i-particles per AVX2 vector with rsqrt_4th (the reference's vector build)."""
     i_off, j_off, j_idx = _walk_args(i_off, j_off, j_idx)
     keep = []
 
@@ -228,8 +242,9 @@ def gravity(i_off, j_off, j_idx, pj: dict, pi: dict, s_off=None, s_idx=None, ps:
     ni = int(i_off[-1])
     out = {k: np.zeros(ni) for k in ("ax", "ay", "az", "pot", "eta")}
     o = GravOut(**{k: _ptr(v) for k, v in out.items()})
-    lib().orc_gravity(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pj),
-                      _ptr(s_off), _ptr(s_idx), src(ps), src(pi), o, int(nthreads))
+    fn = _simd().orc_gravity_simd if simd else lib().orc_gravity
+    fn(len(i_off) - 1, _ptr(i_off), _ptr(j_off), _ptr(j_idx), src(pj),
+       _ptr(s_off), _ptr(s_idx), src(ps), src(pi), o, int(nthreads))
     return out
 
 

@@ -89,6 +89,23 @@ void orc_density_avx2(int n_walk, const int32_t *i_off, const int32_t *j_off, co
 /* rcp_4th / rsqrt_4th of vector_x86.hpp:501-534, one lane. */
 double orc_rcp_4th(double x);
 double orc_rsqrt_4th(double x);
+/* orc_density_avx2 computed four i-particles per AVX2 vector, as the
+ * reference's USE_INTRINSICS calcDensity runs (hdr_density.hpp:82-242):
+ * bitwise the same results (sphk_oracle_simd.c).  For timing the CPU
+ * baseline on the vector path.  Requires AVX2 (orc_density_simd_supported). */
+void orc_density_simd(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                      orc_dens_src pi, orc_dens_src pj, double eta, double ksrh, double hmax,
+                      orc_dens_out out, int nthreads);
+int orc_density_simd_supported(void);
+/* orc_hydro and orc_gravity four i-particles per AVX2 vector with the
+ * USE_INTRINSICS builds' refined reciprocals (hdr_hydro.hpp:128-311,
+ * hdr_gravity.hpp:194-291): the CPU baseline's vector path. */
+void orc_hydro_simd(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,
+                    orc_hydro_src pi, orc_hydro_src pj, orc_hydro_out out, int nthreads);
+void orc_gravity_simd(int n_walk, const int32_t *i_off,
+                      const int32_t *j_off, const int32_t *j_idx, orc_grav_src pj,
+                      const int32_t *s_off, const int32_t *s_idx, orc_grav_src ps,
+                      orc_grav_src pi, orc_grav_out out, int nthreads);
 
 /* Scalar calcHydro, hdr_hydro.hpp:312-424.  Outputs are assigned. */
 void orc_hydro(int n_walk, const int32_t *i_off, const int32_t *j_off, const int32_t *j_idx,

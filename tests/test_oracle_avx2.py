@@ -11,9 +11,19 @@ series (rcp_4th, rsqrt_4th, vector_x86.hpp:501-534).
     scalar functor's density and H to 1e-14 and Omega^-1, div v and |curl v|
     to 1e-12 of their scale, with identical neighbour counts: the shipped
     CPU path and the scalar oracle agree to FP64 rounding, so the scalar
-    restatement stands for both.
+    restatement stands for both;
+  * the vectorised form of that arithmetic (four i-particles per AVX2
+    vector, as the shipped branch runs; the CPU baseline's density) equals
+    the one-lane restatement bit for bit, also for walks whose particle
+    count is not a multiple of four;
+  * the vector hydro and gravity functors of the CPU baseline (the shipped
+    USE_INTRINSICS branches, hdr_hydro.hpp:128-311, hdr_gravity.hpp:194-291)
+    agree with the scalar functors to 1e-11 of each field's scale (rcp_4th
+    is within 2e-14 of 1/x per call; diffu sums terms of both signs), with
+    conduction (alpha_u > 0) and the SPJ lists live.
 """
 import numpy as np
+import pytest
 
 import oracle
 from tests._cases import polytrope_case, scaled_err
@@ -40,3 +50,37 @@ def test_avx2_density_matches_scalar():
         assert scaled_err(b[k], a[k]) < 1e-14, k
     for k in ("grdh", "divv", "rotv"):
         assert scaled_err(b[k], a[k], 1e-1) < 1e-12, k
+
+
+@pytest.mark.skipif(not oracle.lib().orc_density_simd_supported(), reason="host CPU without AVX2")
+def test_vector_density_equals_one_lane_restatement():
+    case = polytrope_case(3000, seed=1003, n_group=32)
+    p = case["p"]
+    ni = np.diff(case["i_off"])
+    assert (ni % 4 != 0).any() and (ni % 4 == 0).any()
+    a = oracle.density(case["i_off"], *case["dens_lists"], p, p, eta=case["eta"], hmax=case["hmax"],
+                       avx2=True)
+    b = oracle.density(case["i_off"], *case["dens_lists"], p, p, eta=case["eta"], hmax=case["hmax"],
+                       simd=True)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.skipif(not oracle.lib().orc_density_simd_supported(), reason="host CPU without AVX2")
+def test_vector_hydro_and_gravity_match_scalar():
+    case = polytrope_case(3000, seed=1004, n_group=32)
+    hs = dict(case["hydro_src"])
+    hs["alphu"] = np.random.default_rng(4).uniform(0.1, 1.0, len(hs["m"]))
+    io, (jo, ji) = case["hydro_i_off"], case["hydro_lists"]
+    a = oracle.hydro(io, jo, ji, hs, hs)
+    b = oracle.hydro(io, jo, ji, hs, hs, simd=True)
+    assert np.abs(a["diffu"]).max() > 0 and np.abs(a["udot"]).max() > 0
+    for k in a:
+        assert scaled_err(b[k], a[k]) < 1e-11, k
+    jl, g = case["grav_lists"], case["grav_src"]
+    assert len(jl[3]) > 0
+    a = oracle.gravity(case["grav_i_off"], jl[0], jl[1], g, g, s_off=jl[2], s_idx=jl[3], ps=case["grav_mom"])
+    b = oracle.gravity(case["grav_i_off"], jl[0], jl[1], g, g, s_off=jl[2], s_idx=jl[3], ps=case["grav_mom"],
+                       simd=True)
+    for k in a:
+        assert scaled_err(b[k], a[k]) < 1e-12, k
