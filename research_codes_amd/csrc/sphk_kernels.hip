@@ -1091,33 +1091,37 @@ __device__ __forceinline__ void gravity_pair_rel(const float4 &p, float xi, floa
     if (ETA) a.et += w;
 }
 
+// Records are staged kGravBlock at a time (one barrier pair per block, four
+// records in flight per lane); each lane's FP32 partial sums over a block
+// (at most kGravBlock / S terms) are folded into FP64 once per block.
+constexpr int kGravBlock = 4 * kTile;
+
 template <bool ETA>
 __device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb, int ke,
                                                  const int32_t *__restrict__ idx, const sphk_grav_src &src,
                                                  double cx, double cy, double cz, float xi, float yi,
                                                  float zi, float e2, float4 *sp, double (&acc)[5])
 {
-    for (int t0 = kb; t0 < ke; t0 += kTile) {
-        const int nt = min(kTile, ke - t0);
+    const int k0 = M.slice < M.S ? M.slice : kGravBlock;   // idle lanes: empty sweeps
+    for (int b0 = kb; b0 < ke; b0 += kGravBlock) {
+        const int nb = min(kGravBlock, ke - b0);
         __syncthreads();
-        if (lane < nt) {
-            const int j = idx ? idx[t0 + lane] : t0 + lane;
-            sp[lane] = make_float4((float)(src.x[j] - cx), (float)(src.y[j] - cy), (float)(src.z[j] - cz),
-                                   (float)src.m[j]);
-        } else {
-            sp[lane] = make_float4(1e15f, 1e15f, 1e15f, 0.f);
+        for (int e = lane; e < nb; e += kWave) {
+            const int j = idx ? idx[b0 + e] : b0 + e;
+            sp[e] = make_float4((float)(src.x[j] - cx), (float)(src.y[j] - cy), (float)(src.z[j] - cz),
+                                (float)src.m[j]);
         }
         __syncthreads();
         GravAcc<float> a{0.f, 0.f, 0.f, 0.f, 0.f};
-        int k = M.slice;
-        for (; k + 3 * M.S < nt; k += 4 * M.S) {
+        int k = k0;
+        for (; k + 3 * M.S < nb; k += 4 * M.S) {
             const float4 p0 = sp[k], p1 = sp[k + M.S], p2 = sp[k + 2 * M.S], p3 = sp[k + 3 * M.S];
             gravity_pair_rel<ETA>(p0, xi, yi, zi, e2, a);
             gravity_pair_rel<ETA>(p1, xi, yi, zi, e2, a);
             gravity_pair_rel<ETA>(p2, xi, yi, zi, e2, a);
             gravity_pair_rel<ETA>(p3, xi, yi, zi, e2, a);
         }
-        for (; k < nt; k += M.S) gravity_pair_rel<ETA>(sp[k], xi, yi, zi, e2, a);
+        for (; k < nb; k += M.S) gravity_pair_rel<ETA>(sp[k], xi, yi, zi, e2, a);
         acc[0] += (double)a.ax;
         acc[1] += (double)a.ay;
         acc[2] += (double)a.az;
@@ -1135,8 +1139,10 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
     const int32_t *__restrict__ s_idx, sphk_grav_src ps, sphk_grav_src pi, sphk_grav_out out)
 {
     constexpr int kTileBytes = kTile * sizeof(GRec<MIXED>);
+    constexpr int kRelBytes = (MIXED && UNIFORM) ? kGravBlock * (int)sizeof(float4) : 0;
     constexpr int kRedBytes = 5 * kWave * sizeof(double);
-    __shared__ __attribute__((aligned(32))) char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
+    constexpr int kMax1 = kTileBytes > kRedBytes ? kTileBytes : kRedBytes;
+    __shared__ __attribute__((aligned(32))) char smem[kMax1 > kRelBytes ? kMax1 : kRelBytes];
     GRec<MIXED> *sp = reinterpret_cast<GRec<MIXED> *>(smem);
     double *red = reinterpret_cast<double *>(smem);
 
