@@ -335,6 +335,22 @@ def ensemble_main(args) -> int:
     graph = s.capture_force(MIXED)
     ms_graph = timed(graph.replay, args.steps)
     del graph
+    # the same step with persistent launches (grid-resident workgroups taking
+    # walks of all runs from per-XCD device queues), eager and captured
+    for k, v in snap.items():
+        s.s[k].copy_(v)
+    s.force(MIXED)
+    ref = {k: s.s[k].clone() for k in ("dens", "ax", "ay", "az", "udot")}
+    ctx.set_params(persistent=True)
+    ms_persistent = timed(lambda: s.force(MIXED), args.steps)
+    same = all(torch.equal(s.s[k], v) for k, v in ref.items())
+    graph = s.capture_force(MIXED)
+    ms_persistent_graph = timed(graph.replay, args.steps)
+    same = same and all(torch.equal(s.s[k], v) for k, v in ref.items())
+    del graph
+    ctx.set_params(persistent=False)
+    log(f"persistent launches: {ms_persistent:.3f} ms eager, {ms_persistent_graph:.3f} ms replayed "
+        f"(ordinary {ms_batched:.3f} / {ms_graph:.3f}); outputs bitwise equal: {same}")
     # the same walks launched run by run (one launch per functor per run)
     full = dict(s.lists)
     per_run = []
@@ -370,6 +386,8 @@ def ensemble_main(args) -> int:
            "config": {"workload": "ensemble of 64 parabolic collisions, N_run log-uniform in [1e4, 2e5], "
                                   "q ~ U(0.3,1), b ~ U(0,1)", "N": s.n, "runs": len(counts), "n_group": g},
            "ms_per_step_hipgraph": round(ms_graph, 4),
+           "persistent": {"ms_per_step": round(ms_persistent, 4), "ms_per_step_hipgraph": round(ms_persistent_graph, 4),
+                          "bitwise_equal": same},
            "ms_per_step_run_by_run": round(ms_seq, 4), "speedup_batched": round(ms_seq / ms_batched, 3),
            "entries": entries}
     print(json.dumps(out), flush=True)

@@ -82,6 +82,14 @@ typedef struct {
      * tiles.  192 needs less LDS (more waves per CU), 256 keeps more walks
      * resident.  0 (default): 256. */
     int list_cap;
+    /* Launch of the three functors.  0 (default): one workgroup per walk.
+     * 1: persistent -- as many workgroups as the chip holds at once, each
+     * taking walks from a device queue of its XCD until none is left (one
+     * launch for a whole ensemble of runs, any imbalance between them).
+     * Results are bitwise those of 0.  Each launch uses the next of 64 walk
+     * queues of the context: at most 64 persistent launches of one context
+     * may be in flight at once. */
+    int persistent;
 } sphk_params;
 
 /* ---- inputs / outputs (all FP64 unless stated) ------------------------ */

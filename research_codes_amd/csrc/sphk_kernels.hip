@@ -45,6 +45,7 @@ constexpr double kFar = 1e10;
 // Walks past the last whole round of chunks keep their own index.
 constexpr int kXcd = 8;
 constexpr int kXcdChunk = 64;
+static_assert(kQueueInts == (kXcd + 1) * kQueueStride, "walk queue layout");
 
 __device__ __forceinline__ int xcd_walk(int b, int n_walk)
 {
@@ -52,6 +53,40 @@ __device__ __forceinline__ int xcd_walk(int b, int n_walk)
     if (b >= n_full) return b;
     const int x = b % kXcd, k = b / kXcd;
     return ((k / kXcdChunk) * kXcd + x) * kXcdChunk + k % kXcdChunk;
+}
+
+// Walk scheduling.  Ordinary launch: one workgroup per walk (grid = n_walk),
+// workgroup b takes walk xcd_walk(b).  Persistent launch (a grid sized to the
+// resident capacity of the chip): every workgroup repeatedly claims the next
+// virtual index v = x + 8 k of its own XCD x (counter k of that XCD's queue)
+// and runs walk xcd_walk(v), so an XCD works through the same walks as in
+// the ordinary launch; it stops at the first v >= n_walk.  The last
+// workgroup to finish zeroes the queue for its next use.  Queue layout:
+// kXcd claim counters and one exit counter, kQueueStride ints apart.
+template <bool PERSIST, typename F>
+__device__ __forceinline__ void run_walks(int n_walk, int *queue, F &&walk)
+{
+    if (!PERSIST) {
+        if ((int)blockIdx.x < n_walk) walk(xcd_walk(blockIdx.x, n_walk), (int)threadIdx.x);
+        return;
+    }
+    const int x = (int)(blockIdx.x % kXcd);
+    int *ctr = queue + x * kQueueStride;
+    for (;;) {
+        int k = 0;
+        if (threadIdx.x == 0) k = atomicAdd(ctr, 1);
+        const int v = x + kXcd * __shfl(k, 0, kWave);
+        if (v >= n_walk) break;
+        walk(xcd_walk(v, n_walk), (int)threadIdx.x);
+        __syncthreads();   // LDS of this walk no longer read
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(queue + kXcd * kQueueStride, 1) == (int)gridDim.x - 1) {
+            for (int c = 0; c < kXcd; c++) atomicExch(queue + c * kQueueStride, 0);
+            atomicExch(queue + kXcd * kQueueStride, 0);
+        }
+    }
 }
 
 // Lane map of one i-chunk: S = floor(64/ni) slices.
@@ -365,11 +400,11 @@ __device__ void density_streaming(int lane, const Map &M, int i, int jb, int je,
 
 // Launch bound: 3 waves per SIMD (at most 168 VGPRs; a few spill), the
 // occupancy the 192-record LDS layout allows.
-template <bool MIXED, int CAP>
+template <bool MIXED, int CAP, bool PERSIST>
 __global__ __launch_bounds__(kWave, 3) void density_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_dens_src pi, sphk_dens_src pj, sphk_dens_out out,
-    double eta_ksrh, double hmax)
+    double eta_ksrh, double hmax, int *queue)
 {
     using R = real_t<MIXED>;
     static_assert(CAP % kTile == 0 && CAP <= kDensCap, "list capacity");
@@ -385,177 +420,253 @@ __global__ __launch_bounds__(kWave, 3) void density_kernel(
     __shared__ uint16_t s_ci[kWave], s_coff[kWave];
     int *s_cnt = reinterpret_cast<int *>(red);   // membership counts, before any reduction
 
-    if ((int)blockIdx.x >= n_walk) return;
-    const int w = xcd_walk(blockIdx.x, n_walk);
-    const int lane = threadIdx.x;
-    const int ib = i_off[w], ie = i_off[w + 1];
-    const int jb = j_off[w], je = j_off[w + 1];
-    const int nj = je - jb;
-    const int ntile = (nj + kTile - 1) / kTile;
+    auto walk = [&](const int w, const int lane) {
+        const int ib = i_off[w], ie = i_off[w + 1];
+        const int jb = j_off[w], je = j_off[w + 1];
+        const int nj = je - jb;
+        const int ntile = (nj + kTile - 1) / kTile;
 
-    if (nj > CAP) {
-        for (int c0 = ib; c0 < ie; c0 += kWave) {
+        if (nj > CAP) {
+            for (int c0 = ib; c0 < ie; c0 += kWave) {
+                const Map M(min(kWave, ie - c0), lane);
+                density_streaming<MIXED>(lane, M, c0 + M.il, jb, je, j_idx, pi, pj, out, eta_ksrh, hmax,
+                                         sp, sv, red, shs);
+            }
+            return;
+        }
+
+        // one burst: every record of the list, padded to whole tiles, plus the inert record
+        for (int e = lane; e < ntile * kTile; e += kWave) dens_stage<MIXED>(e, jb, je, j_idx, pj, sp, sv, true);
+        if (lane == 0) dens_stage<MIXED>(kDensPad, 0, 0, j_idx, pj, sp, sv, true);
+
+        for (int c0 = ib; c0 < ie;) {
             const Map M(min(kWave, ie - c0), lane);
-            density_streaming<MIXED>(lane, M, c0 + M.il, jb, je, j_idx, pi, pj, out, eta_ksrh, hmax,
-                                     sp, sv, red, shs);
-        }
-        return;
-    }
+            __syncthreads();   // staging done; LDS of the previous chunk no longer read
 
-    // one burst: every record of the list, padded to whole tiles, plus the inert record
-    for (int e = lane; e < ntile * kTile; e += kWave) dens_stage<MIXED>(e, jb, je, j_idx, pj, sp, sv, true);
-    if (lane == 0) dens_stage<MIXED>(kDensPad, 0, 0, j_idx, pj, sp, sv, true);
-
-    for (int c0 = ib; c0 < ie;) {
-        const Map M(min(kWave, ie - c0), lane);
-        __syncthreads();   // staging done; LDS of the previous chunk no longer read
-
-        // -- membership sweep: lane (i, slice) marks its records within R_i
-        uint64_t wd[kWords];
-        int cnt = 0;
-        {
-            const int i = c0 + M.il;
-            const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
-            const double h0 = pi.ksr[i], rs = pi.rs[i];
-            const double rme = (h0 > rs ? h0 : rs) * (1. + 1e-12);
-            const double rm2 = rme * rme;
-#pragma unroll
-            for (int t = 0; t < kWords; t++) {
-                uint64_t b = 0;
-                if (t < ntile) {
-                    const double4 *tp = sp + t * kTile;
-                    const int nt = min(kTile, nj - t * kTile);
-                    int k = M.slice;
-                    for (; k + 3 * M.S < nt; k += 4 * M.S) {
-                        const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
-                        const double x0 = px - j0.x, y0 = py - j0.y, z0 = pz - j0.z;
-                        const double x1 = px - j1.x, y1 = py - j1.y, z1 = pz - j1.z;
-                        const double x2 = px - j2.x, y2 = py - j2.y, z2 = pz - j2.z;
-                        const double x3 = px - j3.x, y3 = py - j3.y, z3 = pz - j3.z;
-                        b |= (uint64_t)(x0 * x0 + y0 * y0 + z0 * z0 < rm2) << k;
-                        b |= (uint64_t)(x1 * x1 + y1 * y1 + z1 * z1 < rm2) << (k + M.S);
-                        b |= (uint64_t)(x2 * x2 + y2 * y2 + z2 * z2 < rm2) << (k + 2 * M.S);
-                        b |= (uint64_t)(x3 * x3 + y3 * y3 + z3 * z3 < rm2) << (k + 3 * M.S);
+            // -- membership sweep: lane (i, slice) marks its records within R_i
+            uint64_t wd[kWords];
+            int cnt = 0;
+            {
+                const int i = c0 + M.il;
+                const double px = pi.x[i], py = pi.y[i], pz = pi.z[i];
+                const double h0 = pi.ksr[i], rs = pi.rs[i];
+                const double rme = (h0 > rs ? h0 : rs) * (1. + 1e-12);
+                const double rm2 = rme * rme;
+    #pragma unroll
+                for (int t = 0; t < kWords; t++) {
+                    uint64_t b = 0;
+                    if (t < ntile) {
+                        const double4 *tp = sp + t * kTile;
+                        const int nt = min(kTile, nj - t * kTile);
+                        int k = M.slice;
+                        for (; k + 3 * M.S < nt; k += 4 * M.S) {
+                            const double4 j0 = tp[k], j1 = tp[k + M.S], j2 = tp[k + 2 * M.S], j3 = tp[k + 3 * M.S];
+                            const double x0 = px - j0.x, y0 = py - j0.y, z0 = pz - j0.z;
+                            const double x1 = px - j1.x, y1 = py - j1.y, z1 = pz - j1.z;
+                            const double x2 = px - j2.x, y2 = py - j2.y, z2 = pz - j2.z;
+                            const double x3 = px - j3.x, y3 = py - j3.y, z3 = pz - j3.z;
+                            b |= (uint64_t)(x0 * x0 + y0 * y0 + z0 * z0 < rm2) << k;
+                            b |= (uint64_t)(x1 * x1 + y1 * y1 + z1 * z1 < rm2) << (k + M.S);
+                            b |= (uint64_t)(x2 * x2 + y2 * y2 + z2 * z2 < rm2) << (k + 2 * M.S);
+                            b |= (uint64_t)(x3 * x3 + y3 * y3 + z3 * z3 < rm2) << (k + 3 * M.S);
+                        }
+                        for (; k < nt; k += M.S) {
+                            const double dx = px - tp[k].x, dy = py - tp[k].y, dz = pz - tp[k].z;
+                            b |= (uint64_t)(dx * dx + dy * dy + dz * dz < rm2) << k;
+                        }
                     }
-                    for (; k < nt; k += M.S) {
-                        const double dx = px - tp[k].x, dy = py - tp[k].y, dz = pz - tp[k].z;
-                        b |= (uint64_t)(dx * dx + dy * dy + dz * dz < rm2) << k;
+                    wd[t] = b;
+                    cnt += __popcll(b);
+                }
+                s_cnt[lane] = cnt;
+            }
+            __syncthreads();
+            int start = 0, ci = 0;
+            if (M.slice < M.S)
+                for (int s2 = 0; s2 < M.slice; s2++) start += s_cnt[M.il + M.ni * s2];
+            if (M.lead)
+                for (int s2 = 0; s2 < M.S; s2++) ci += s_cnt[M.il + M.ni * s2];
+            // take the leading i-particles whose lists fit in kDensComp entries
+            // (every single list fits: c_i <= kDensCap)
+            const int incl = wave_scan_incl(ci, lane);   // leads are lanes 0 .. ni-1
+            const bool fits = M.lead && incl <= kDensComp;
+            const int n_take = __popcll(__ballot(fits));
+            const int total = __shfl(incl, n_take - 1, kWave);
+            const bool take = M.il < n_take;             // lane works on a taken i
+
+            if (M.lead && take) {
+                s_coff[M.il] = (uint16_t)(incl - ci);
+                s_ci[M.il] = (uint16_t)ci;
+            }
+            s_own[lane] = -1;
+            __syncthreads();
+            if (take && M.slice < M.S) {
+                int o = s_coff[M.il] + start;
+    #pragma unroll
+                for (int t = 0; t < kWords; t++) {
+                    uint64_t b = wd[t];
+                    while (b) {
+                        cidx[o++] = (uint8_t)(t * kTile + __builtin_ctzll(b));
+                        b &= b - 1;
                     }
                 }
-                wd[t] = b;
-                cnt += __popcll(b);
             }
-            s_cnt[lane] = cnt;
-        }
-        __syncthreads();
-        int start = 0, ci = 0;
-        if (M.slice < M.S)
-            for (int s2 = 0; s2 < M.slice; s2++) start += s_cnt[M.il + M.ni * s2];
-        if (M.lead)
-            for (int s2 = 0; s2 < M.S; s2++) ci += s_cnt[M.il + M.ni * s2];
-        // take the leading i-particles whose lists fit in kDensComp entries
-        // (every single list fits: c_i <= kDensCap)
-        const int incl = wave_scan_incl(ci, lane);   // leads are lanes 0 .. ni-1
-        const bool fits = M.lead && incl <= kDensComp;
-        const int n_take = __popcll(__ballot(fits));
-        const int total = __shfl(incl, n_take - 1, kWave);
-        const bool take = M.il < n_take;             // lane works on a taken i
+            // lanes per taken i: max(1, ceil(c_i / T)), T = ceil(C / (64 - n)); they sum to <= 64
+            const int tsteps = n_take < kWave ? max(1, (total + (kWave - n_take) - 1) / (kWave - n_take)) : 1;
+            const int si = (M.lead && take) ? (n_take < kWave ? max(1, (ci + tsteps - 1) / tsteps) : 1) : 0;
+            const int lincl = wave_scan_incl(si, lane);
+            if (si > 0)
+                for (int s2 = 0; s2 < si; s2++) s_own[lincl - si + s2] = M.il | (s2 << 8) | (si << 16);
+            __syncthreads();
+            const int own = s_own[lane];
 
-        if (M.lead && take) {
-            s_coff[M.il] = (uint16_t)(incl - ci);
-            s_ci[M.il] = (uint16_t)ci;
-        }
-        s_own[lane] = -1;
-        __syncthreads();
-        if (take && M.slice < M.S) {
-            int o = s_coff[M.il] + start;
-#pragma unroll
-            for (int t = 0; t < kWords; t++) {
-                uint64_t b = wd[t];
-                while (b) {
-                    cidx[o++] = (uint8_t)(t * kTile + __builtin_ctzll(b));
-                    b &= b - 1;
-                }
+            // lane's particle and compacted range; every lane runs the wave's
+            // largest step count, idle steps read the inert record
+            int my = own >= 0 ? (own & 0xff) : 0;
+            int e0 = 0, n_e = 0, stride = 1;
+            const uint8_t *myidx = cidx;
+            if (own >= 0) {
+                e0 = (own >> 8) & 0xff;
+                stride = own >> 16;
+                n_e = s_ci[my];
+                myidx = cidx + s_coff[my];
             }
-        }
-        // lanes per taken i: max(1, ceil(c_i / T)), T = ceil(C / (64 - n)); they sum to <= 64
-        const int tsteps = n_take < kWave ? max(1, (total + (kWave - n_take) - 1) / (kWave - n_take)) : 1;
-        const int si = (M.lead && take) ? (n_take < kWave ? max(1, (ci + tsteps - 1) / tsteps) : 1) : 0;
-        const int lincl = wave_scan_incl(si, lane);
-        if (si > 0)
-            for (int s2 = 0; s2 < si; s2++) s_own[lincl - si + s2] = M.il | (s2 << 8) | (si << 16);
-        __syncthreads();
-        const int own = s_own[lane];
+            const int nsteps = wave_max(n_e > e0 ? (n_e - e0 + stride - 1) / stride : 0);
+            bool compact = true;
+            bool owner = own >= 0 && e0 == 0;
+            DensI I = dens_load_i(pi, c0 + my);
+            const Map MF(n_take, lane);   // full sweeps over the taken i-particles
+            double hs = I.ksr;
+            double dens = 0.;
+            long long np = 0;
 
-        // lane's particle and compacted range; every lane runs the wave's
-        // largest step count, idle steps read the inert record
-        int my = own >= 0 ? (own & 0xff) : 0;
-        int e0 = 0, n_e = 0, stride = 1;
-        const uint8_t *myidx = cidx;
-        if (own >= 0) {
-            e0 = (own >> 8) & 0xff;
-            stride = own >> 16;
-            n_e = s_ci[my];
-            myidx = cidx + s_coff[my];
-        }
-        const int nsteps = wave_max(n_e > e0 ? (n_e - e0 + stride - 1) / stride : 0);
-        bool compact = true;
-        bool owner = own >= 0 && e0 == 0;
-        DensI I = dens_load_i(pi, c0 + my);
-        const Map MF(n_take, lane);   // full sweeps over the taken i-particles
-        double hs = I.ksr;
-        double dens = 0.;
-        long long np = 0;
-
-        // -- three fixed-point passes: rho(H_k), n(H_k), H_{k+1} (hdr_density.hpp:264-302).
-        // When every lane has at most kDensR members, their exact distances
-        // and masses are held in registers for the passes (one sqrt per
-        // member instead of one rsqrt refinement per member and pass); the
-        // passes that remain after an H beyond R_i run below.
-        int rep0 = 0;
-        if (nsteps <= kDensR) {
-            double rr[kDensR], mm[kDensR];
-#pragma unroll
-            for (int t = 0; t < kDensR; t++) {
-                if (t < nsteps) {
-                    const int e = e0 + t * stride;
-                    const double4 j = sp[e < n_e ? myidx[e] : kDensPad];
-                    rr[t] = exact_r(I.x - j.x, I.y - j.y, I.z - j.z);
-                    mm[t] = j.w;
-                }
-            }
-            for (; rep0 < 3; rep0++) {
-                const double hi = 1. / hs;
-                const double hi3 = hi * hi * hi;
-                double rh = 0.;
-                int nq = 0;
-#pragma unroll
+            // -- three fixed-point passes: rho(H_k), n(H_k), H_{k+1} (hdr_density.hpp:264-302).
+            // When every lane has at most kDensR members, their exact distances
+            // and masses are held in registers for the passes (one sqrt per
+            // member instead of one rsqrt refinement per member and pass); the
+            // passes that remain after an H beyond R_i run below.
+            int rep0 = 0;
+            if (nsteps <= kDensR) {
+                double rr[kDensR], mm[kDensR];
+    #pragma unroll
                 for (int t = 0; t < kDensR; t++) {
                     if (t < nsteps) {
-                        const double q = rr[t] * hi;
-                        rh += mm[t] * hi3 * kernel0th(q);
-                        nq += (q < 1.) ? 1 : 0;
+                        const int e = e0 + t * stride;
+                        const double4 j = sp[e < n_e ? myidx[e] : kDensPad];
+                        rr[t] = exact_r(I.x - j.x, I.y - j.y, I.z - j.z);
+                        mm[t] = j.w;
                     }
                 }
-                __syncthreads();
-                red[lane] = rh;
-                red[kWave + lane] = (double)nq;
-                __syncthreads();
-                double v0 = red[lane], v1 = red[kWave + lane];
-                if (owner)
-                    for (int t = 1; t < stride; t++) {
-                        v0 += red[lane + t];
-                        v1 += red[kWave + lane + t];
+                for (; rep0 < 3; rep0++) {
+                    const double hi = 1. / hs;
+                    const double hi3 = hi * hi * hi;
+                    double rh = 0.;
+                    int nq = 0;
+    #pragma unroll
+                    for (int t = 0; t < kDensR; t++) {
+                        if (t < nsteps) {
+                            const double q = rr[t] * hi;
+                            rh += mm[t] * hi3 * kernel0th(q);
+                            nq += (q < 1.) ? 1 : 0;
+                        }
                     }
+                    __syncthreads();
+                    red[lane] = rh;
+                    red[kWave + lane] = (double)nq;
+                    __syncthreads();
+                    double v0 = red[lane], v1 = red[kWave + lane];
+                    if (owner)
+                        for (int t = 1; t < stride; t++) {
+                            v0 += red[lane + t];
+                            v1 += red[kWave + lane + t];
+                        }
+                    int viol = 0;
+                    if (owner) {
+                        dens = v0;
+                        np = (long long)v1;
+                        const double hnew = dens_next_h(eta_ksrh, I.m, dens, hmax);
+                        shs[my] = hnew;
+                        viol = hnew > (I.ksr > I.rs ? I.ksr : I.rs);
+                    }
+                    if (__syncthreads_or(viol)) {   // full sweeps from the next pass on
+                        if (owner) {
+                            red[my] = dens;
+                            red[kWave + my] = (double)np;
+                        }
+                        __syncthreads();
+                        compact = false;
+                        my = MF.il;
+                        owner = MF.lead;
+                        I = dens_load_i(pi, c0 + my);
+                        dens = red[my];
+                        np = (long long)red[kWave + my];
+                        hs = shs[my];
+                        rep0++;
+                        break;
+                    }
+                    hs = shs[my];
+                }
+            }
+            for (int rep = rep0; rep < 3; rep++) {
+                const double hi = 1. / hs;
+                const double hi3 = hi * hi * hi;
+                Pass1 a{0., 0};
+                double v[2];
+                if (compact) {
+                    int st = 0, e = e0;
+                    for (; st + 4 <= nsteps; st += 4, e += 4 * stride) {
+                        const int k0 = e < n_e ? myidx[e] : kDensPad;
+                        const int k1 = e + stride < n_e ? myidx[e + stride] : kDensPad;
+                        const int k2 = e + 2 * stride < n_e ? myidx[e + 2 * stride] : kDensPad;
+                        const int k3 = e + 3 * stride < n_e ? myidx[e + 3 * stride] : kDensPad;
+                        const double4 j0 = sp[k0], j1 = sp[k1], j2 = sp[k2], j3 = sp[k3];
+                        dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
+                        dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
+                        dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
+                        dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
+                    }
+                    for (; st < nsteps; st++, e += stride)
+                        dens_pass1_pair(sp[e < n_e ? myidx[e] : kDensPad], I.x, I.y, I.z, hi, hi3, a);
+                    __syncthreads();
+                    red[lane] = a.rh;
+                    red[kWave + lane] = (double)a.nj;
+                    __syncthreads();
+                    v[0] = red[lane];
+                    v[1] = red[kWave + lane];
+                    if (owner)
+                        for (int t = 1; t < stride; t++) {
+                            v[0] += red[lane + t];
+                            v[1] += red[kWave + lane + t];
+                        }
+                } else {
+                    for (int t = 0; t < ntile; t++) {
+                        const double4 *tp = sp + t * kTile;
+                        const int nt = min(kTile, nj - t * kTile);
+                        int k = MF.slice;
+                        for (; k + 3 * MF.S < nt; k += 4 * MF.S) {
+                            const double4 j0 = tp[k], j1 = tp[k + MF.S], j2 = tp[k + 2 * MF.S], j3 = tp[k + 3 * MF.S];
+                            dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
+                            dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
+                            dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
+                            dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
+                        }
+                        for (; k < nt; k += MF.S) dens_pass1_pair(tp[k], I.x, I.y, I.z, hi, hi3, a);
+                    }
+                    v[0] = a.rh;
+                    v[1] = (double)a.nj;
+                    slice_reduce<2>(v, red, MF, lane);
+                }
                 int viol = 0;
                 if (owner) {
-                    dens = v0;
-                    np = (long long)v1;
+                    dens = v[0];
+                    np = (long long)v[1];
                     const double hnew = dens_next_h(eta_ksrh, I.m, dens, hmax);
                     shs[my] = hnew;
-                    viol = hnew > (I.ksr > I.rs ? I.ksr : I.rs);
+                    viol = compact && hnew > (I.ksr > I.rs ? I.ksr : I.rs);
                 }
-                if (__syncthreads_or(viol)) {   // full sweeps from the next pass on
+                // an H beyond R_i leaves the compacted lists incomplete: full
+                // sweeps from here on, the lead lanes of MF take over ownership
+                if (__syncthreads_or(viol)) {
                     if (owner) {
                         red[my] = dens;
                         red[kWave + my] = (double)np;
@@ -567,158 +678,82 @@ __global__ __launch_bounds__(kWave, 3) void density_kernel(
                     I = dens_load_i(pi, c0 + my);
                     dens = red[my];
                     np = (long long)red[kWave + my];
-                    hs = shs[my];
-                    rep0++;
-                    break;
                 }
                 hs = shs[my];
             }
-        }
-        for (int rep = rep0; rep < 3; rep++) {
-            const double hi = 1. / hs;
-            const double hi3 = hi * hi * hi;
-            Pass1 a{0., 0};
-            double v[2];
+
+            // -- grad-h, divergence, curl with H_3 (hdr_density.hpp:304-349)
+            const double hi_d = 1. / hs;
+            const R hi = (R)hi_d;
+            const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
+            const R vx = (R)I.vx, vy = (R)I.vy, vz = (R)I.vz;
+            double acc[5] = {0., 0., 0., 0., 0.};
             if (compact) {
+                // pair terms summed in R over at most 16 entries, then folded
+                // into FP64 (as the full sweep folds each 64-record tile)
+                Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+                double part[5] = {0., 0., 0., 0., 0.};
+                auto fold = [&]() {
+                    part[0] += (double)a.g;
+                    part[1] += (double)a.dv;
+                    part[2] += (double)a.rx;
+                    part[3] += (double)a.ry;
+                    part[4] += (double)a.rz;
+                    a = Pass2<R>{R(0), R(0), R(0), R(0), R(0)};
+                };
                 int st = 0, e = e0;
-                for (; st + 4 <= nsteps; st += 4, e += 4 * stride) {
+                for (; st + 2 <= nsteps; st += 2, e += 2 * stride) {
                     const int k0 = e < n_e ? myidx[e] : kDensPad;
                     const int k1 = e + stride < n_e ? myidx[e + stride] : kDensPad;
-                    const int k2 = e + 2 * stride < n_e ? myidx[e + 2 * stride] : kDensPad;
-                    const int k3 = e + 3 * stride < n_e ? myidx[e + 3 * stride] : kDensPad;
-                    const double4 j0 = sp[k0], j1 = sp[k1], j2 = sp[k2], j3 = sp[k3];
-                    dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
-                    dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
-                    dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
-                    dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
+                    dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                    dens_pass2_pair<R>(sp[k1], sv[k1], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                    if ((st & 14) == 14) fold();
                 }
-                for (; st < nsteps; st++, e += stride)
-                    dens_pass1_pair(sp[e < n_e ? myidx[e] : kDensPad], I.x, I.y, I.z, hi, hi3, a);
-                __syncthreads();
-                red[lane] = a.rh;
-                red[kWave + lane] = (double)a.nj;
-                __syncthreads();
-                v[0] = red[lane];
-                v[1] = red[kWave + lane];
-                if (owner)
-                    for (int t = 1; t < stride; t++) {
-                        v[0] += red[lane + t];
-                        v[1] += red[kWave + lane + t];
-                    }
+                if (st < nsteps) {
+                    const int k0 = e < n_e ? myidx[e] : kDensPad;
+                    dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                }
+                fold();
+                // per-i sums over the i's lanes in FP64, two quantities per round
+                for (int q0 = 0; q0 < 5; q0 += 2) {
+                    const int nq = q0 + 1 < 5 ? 2 : 1;
+                    __syncthreads();
+                    for (int q = 0; q < nq; q++) red[q * kWave + lane] = part[q0 + q];
+                    __syncthreads();
+                    if (owner)
+                        for (int q = 0; q < nq; q++) {
+                            double s = red[q * kWave + lane];
+                            for (int t = 1; t < stride; t++) s += red[q * kWave + lane + t];
+                            acc[q0 + q] = s;
+                        }
+                }
             } else {
                 for (int t = 0; t < ntile; t++) {
                     const double4 *tp = sp + t * kTile;
-                    const int nt = min(kTile, nj - t * kTile);
+                    const R4<R> *tv = sv + t * kTile;
+                    Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
+                    const int nt = min(kTile, nj - t * kTile);   // records of this tile
                     int k = MF.slice;
-                    for (; k + 3 * MF.S < nt; k += 4 * MF.S) {
-                        const double4 j0 = tp[k], j1 = tp[k + MF.S], j2 = tp[k + 2 * MF.S], j3 = tp[k + 3 * MF.S];
-                        dens_pass1_pair(j0, I.x, I.y, I.z, hi, hi3, a);
-                        dens_pass1_pair(j1, I.x, I.y, I.z, hi, hi3, a);
-                        dens_pass1_pair(j2, I.x, I.y, I.z, hi, hi3, a);
-                        dens_pass1_pair(j3, I.x, I.y, I.z, hi, hi3, a);
+                    for (; k + MF.S < nt; k += 2 * MF.S) {
+                        dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                        dens_pass2_pair<R>(tp[k + MF.S], tv[k + MF.S], I.x, I.y, I.z, vx, vy, vz, I.id, hi,
+                                           hi4, a);
                     }
-                    for (; k < nt; k += MF.S) dens_pass1_pair(tp[k], I.x, I.y, I.z, hi, hi3, a);
+                    for (; k < nt; k += MF.S)
+                        dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
+                    acc[0] += (double)a.g;
+                    acc[1] += (double)a.dv;
+                    acc[2] += (double)a.rx;
+                    acc[3] += (double)a.ry;
+                    acc[4] += (double)a.rz;
                 }
-                v[0] = a.rh;
-                v[1] = (double)a.nj;
-                slice_reduce<2>(v, red, MF, lane);
+                dens_reduce_grad<MIXED>(acc, red, MF, lane);
             }
-            int viol = 0;
-            if (owner) {
-                dens = v[0];
-                np = (long long)v[1];
-                const double hnew = dens_next_h(eta_ksrh, I.m, dens, hmax);
-                shs[my] = hnew;
-                viol = compact && hnew > (I.ksr > I.rs ? I.ksr : I.rs);
-            }
-            // an H beyond R_i leaves the compacted lists incomplete: full
-            // sweeps from here on, the lead lanes of MF take over ownership
-            if (__syncthreads_or(viol)) {
-                if (owner) {
-                    red[my] = dens;
-                    red[kWave + my] = (double)np;
-                }
-                __syncthreads();
-                compact = false;
-                my = MF.il;
-                owner = MF.lead;
-                I = dens_load_i(pi, c0 + my);
-                dens = red[my];
-                np = (long long)red[kWave + my];
-            }
-            hs = shs[my];
+            if (owner) dens_finish(out, c0 + my, acc, dens, np, hs, I.rs);
+            c0 += n_take;
         }
-
-        // -- grad-h, divergence, curl with H_3 (hdr_density.hpp:304-349)
-        const double hi_d = 1. / hs;
-        const R hi = (R)hi_d;
-        const R hi4 = (R)(hi_d * (hi_d * hi_d * hi_d));
-        const R vx = (R)I.vx, vy = (R)I.vy, vz = (R)I.vz;
-        double acc[5] = {0., 0., 0., 0., 0.};
-        if (compact) {
-            // pair terms summed in R over at most 16 entries, then folded
-            // into FP64 (as the full sweep folds each 64-record tile)
-            Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
-            double part[5] = {0., 0., 0., 0., 0.};
-            auto fold = [&]() {
-                part[0] += (double)a.g;
-                part[1] += (double)a.dv;
-                part[2] += (double)a.rx;
-                part[3] += (double)a.ry;
-                part[4] += (double)a.rz;
-                a = Pass2<R>{R(0), R(0), R(0), R(0), R(0)};
-            };
-            int st = 0, e = e0;
-            for (; st + 2 <= nsteps; st += 2, e += 2 * stride) {
-                const int k0 = e < n_e ? myidx[e] : kDensPad;
-                const int k1 = e + stride < n_e ? myidx[e + stride] : kDensPad;
-                dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-                dens_pass2_pair<R>(sp[k1], sv[k1], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-                if ((st & 14) == 14) fold();
-            }
-            if (st < nsteps) {
-                const int k0 = e < n_e ? myidx[e] : kDensPad;
-                dens_pass2_pair<R>(sp[k0], sv[k0], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-            }
-            fold();
-            // per-i sums over the i's lanes in FP64, two quantities per round
-            for (int q0 = 0; q0 < 5; q0 += 2) {
-                const int nq = q0 + 1 < 5 ? 2 : 1;
-                __syncthreads();
-                for (int q = 0; q < nq; q++) red[q * kWave + lane] = part[q0 + q];
-                __syncthreads();
-                if (owner)
-                    for (int q = 0; q < nq; q++) {
-                        double s = red[q * kWave + lane];
-                        for (int t = 1; t < stride; t++) s += red[q * kWave + lane + t];
-                        acc[q0 + q] = s;
-                    }
-            }
-        } else {
-            for (int t = 0; t < ntile; t++) {
-                const double4 *tp = sp + t * kTile;
-                const R4<R> *tv = sv + t * kTile;
-                Pass2<R> a{R(0), R(0), R(0), R(0), R(0)};
-                const int nt = min(kTile, nj - t * kTile);   // records of this tile
-                int k = MF.slice;
-                for (; k + MF.S < nt; k += 2 * MF.S) {
-                    dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-                    dens_pass2_pair<R>(tp[k + MF.S], tv[k + MF.S], I.x, I.y, I.z, vx, vy, vz, I.id, hi,
-                                       hi4, a);
-                }
-                for (; k < nt; k += MF.S)
-                    dens_pass2_pair<R>(tp[k], tv[k], I.x, I.y, I.z, vx, vy, vz, I.id, hi, hi4, a);
-                acc[0] += (double)a.g;
-                acc[1] += (double)a.dv;
-                acc[2] += (double)a.rx;
-                acc[3] += (double)a.ry;
-                acc[4] += (double)a.rz;
-            }
-            dens_reduce_grad<MIXED>(acc, red, MF, lane);
-        }
-        if (owner) dens_finish(out, c0 + my, acc, dens, np, hs, I.rs);
-        c0 += n_take;
-    }
+    };
+    run_walks<PERSIST>(n_walk, queue, walk);
 }
 
 // ---------------------------------------------------------------------------
@@ -840,10 +875,10 @@ __device__ __forceinline__ HRec<R> hydro_load(const sphk_hydro_src &pj, const in
 
 // FP64: at most 168 VGPRs (3 waves per SIMD, a few spills): 11 % faster than
 // 2 waves without spills.  Mixed needs 115 VGPRs (4 waves) as written.
-template <bool MIXED>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 3))) void hydro_kernel(
+template <bool MIXED, bool PERSIST>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? (PERSIST ? 4 : 1) : 3))) void hydro_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
-    const int32_t *__restrict__ j_idx, sphk_hydro_src pi, sphk_hydro_src pj, sphk_hydro_out out)
+    const int32_t *__restrict__ j_idx, sphk_hydro_src pi, sphk_hydro_src pj, sphk_hydro_out out, int *queue)
 {
     using R = real_t<MIXED>;
     // tile records and the slice-reduction buffer share one LDS block (the
@@ -860,91 +895,91 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
     int32_t *sid = reinterpret_cast<int32_t *>(se + kTile);
     double *red = reinterpret_cast<double *>(smem);
 
-    if ((int)blockIdx.x >= n_walk) return;
-    const int w = xcd_walk(blockIdx.x, n_walk);
-    const int lane = threadIdx.x;
-    const int ib = i_off[w], ie = i_off[w + 1];
-    const int jb = j_off[w], je = j_off[w + 1];
+    auto walk = [&](const int w, const int lane) {
+        const int ib = i_off[w], ie = i_off[w + 1];
+        const int jb = j_off[w], je = j_off[w + 1];
 
-    // mixed: positions as offsets from the walk's first particle
-    const double cx = MIXED ? pi.x[ib] : 0., cy = MIXED ? pi.y[ib] : 0., cz = MIXED ? pi.z[ib] : 0.;
-    for (int c0 = ib; c0 < ie; c0 += kWave) {
-        const Map M(min(kWave, ie - c0), lane);
-        const int i = c0 + M.il;
-        HydroI<R> I;
-        I.px = pi.x[i] - cx;
-        I.py = pi.y[i] - cy;
-        I.pz = pi.z[i] - cz;
-        I.vx = (R)pi.vx[i];
-        I.vy = (R)pi.vy[i];
-        I.vz = (R)pi.vz[i];
-        const double hid = pi.hinv[i];
-        I.hi = (R)hid;
-        I.h5 = (R)(hid * (hid * hid * hid) * hid);
-        I.pp = (R)pi.pres[i];
-        I.pc = (R)pi.presc[i];
-        I.bs = (R)pi.bswt[i];
-        I.dn = (R)pi.dens[i];
-        I.cs = (R)pi.vsnd[i];
-        I.al = (R)pi.alph[i];
-        I.au = (R)pi.alphu[i];
-        I.eg = (R)pi.thrm[i];
-        I.et = (R)pi.eta[i];
-        I.id = pi.id[i];
+        // mixed: positions as offsets from the walk's first particle
+        const double cx = MIXED ? pi.x[ib] : 0., cy = MIXED ? pi.y[ib] : 0., cz = MIXED ? pi.z[ib] : 0.;
+        for (int c0 = ib; c0 < ie; c0 += kWave) {
+            const Map M(min(kWave, ie - c0), lane);
+            const int i = c0 + M.il;
+            HydroI<R> I;
+            I.px = pi.x[i] - cx;
+            I.py = pi.y[i] - cy;
+            I.pz = pi.z[i] - cz;
+            I.vx = (R)pi.vx[i];
+            I.vy = (R)pi.vy[i];
+            I.vz = (R)pi.vz[i];
+            const double hid = pi.hinv[i];
+            I.hi = (R)hid;
+            I.h5 = (R)(hid * (hid * hid * hid) * hid);
+            I.pp = (R)pi.pres[i];
+            I.pc = (R)pi.presc[i];
+            I.bs = (R)pi.bswt[i];
+            I.dn = (R)pi.dens[i];
+            I.cs = (R)pi.vsnd[i];
+            I.al = (R)pi.alph[i];
+            I.au = (R)pi.alphu[i];
+            I.eg = (R)pi.thrm[i];
+            I.et = (R)pi.eta[i];
+            I.id = pi.id[i];
 
-        double acc[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
-        R vsmx = R(0);
-        // each lane loads its record of the tile into registers before the
-        // barrier (loading the next tile's record during the sweep measured
-        // 3 % slower in mixed mode, and its registers spill in FP64)
-        for (int t0 = jb; t0 < je; t0 += kTile) {
-            const int nt = min(kTile, je - t0);
-            const HRec<R> nx = hydro_load<R>(pj, j_idx, t0 + lane, je, cx, cy, cz);
-            __syncthreads();
-            sp[lane] = nx.p;
-            sa[lane] = nx.a;
-            sb[lane] = nx.b;
-            sc[lane] = nx.c;
-            se[lane] = nx.e;
-            sid[lane] = nx.id;
-            __syncthreads();
-            HydroAcc<R> a{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), vsmx};
-            int k = M.slice;
-            for (; k + M.S < nt; k += 2 * M.S) {
-                const int k1 = k + M.S;
-                const PR p0 = sp[k], p1 = sp[k1];
-                const R4<R> a0 = sa[k], a1 = sa[k1], b0 = sb[k], b1 = sb[k1], c0_ = sc[k], c1 = sc[k1];
-                const R e0 = se[k], e1 = se[k1];
-                const int32_t d0 = sid[k], d1 = sid[k1];
-                hydro_pair<R>(p0, a0, b0, c0_, e0, d0, I, a);
-                hydro_pair<R>(p1, a1, b1, c1, e1, d1, I, a);
+            double acc[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+            R vsmx = R(0);
+            // each lane loads its record of the tile into registers before the
+            // barrier (loading the next tile's record during the sweep measured
+            // 3 % slower in mixed mode, and its registers spill in FP64)
+            for (int t0 = jb; t0 < je; t0 += kTile) {
+                const int nt = min(kTile, je - t0);
+                const HRec<R> nx = hydro_load<R>(pj, j_idx, t0 + lane, je, cx, cy, cz);
+                __syncthreads();
+                sp[lane] = nx.p;
+                sa[lane] = nx.a;
+                sb[lane] = nx.b;
+                sc[lane] = nx.c;
+                se[lane] = nx.e;
+                sid[lane] = nx.id;
+                __syncthreads();
+                HydroAcc<R> a{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), vsmx};
+                int k = M.slice;
+                for (; k + M.S < nt; k += 2 * M.S) {
+                    const int k1 = k + M.S;
+                    const PR p0 = sp[k], p1 = sp[k1];
+                    const R4<R> a0 = sa[k], a1 = sa[k1], b0 = sb[k], b1 = sb[k1], c0_ = sc[k], c1 = sc[k1];
+                    const R e0 = se[k], e1 = se[k1];
+                    const int32_t d0 = sid[k], d1 = sid[k1];
+                    hydro_pair<R>(p0, a0, b0, c0_, e0, d0, I, a);
+                    hydro_pair<R>(p1, a1, b1, c1, e1, d1, I, a);
+                }
+                for (; k < nt; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sid[k], I, a);
+                acc[0] += (double)a.ax;
+                acc[1] += (double)a.ay;
+                acc[2] += (double)a.az;
+                acc[3] += (double)a.ud;
+                acc[4] += (double)a.df;
+                acc[5] += (double)a.gx;
+                acc[6] += (double)a.gy;
+                acc[7] += (double)a.gz;
+                vsmx = a.vs;
             }
-            for (; k < nt; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sid[k], I, a);
-            acc[0] += (double)a.ax;
-            acc[1] += (double)a.ay;
-            acc[2] += (double)a.az;
-            acc[3] += (double)a.ud;
-            acc[4] += (double)a.df;
-            acc[5] += (double)a.gx;
-            acc[6] += (double)a.gy;
-            acc[7] += (double)a.gz;
-            vsmx = a.vs;
+            acc[8] = (double)vsmx;
+            slice_reduce<9>(acc, red, M, lane, 1 << 8);
+            // hdr_hydro.hpp:402-419
+            if (M.lead) {
+                out.ax[i] = acc[0] * 0.5;
+                out.ay[i] = acc[1] * 0.5;
+                out.az[i] = acc[2] * 0.5;
+                out.udot[i] = acc[3] * 0.5;
+                out.diffu[i] = acc[4] * 2.;
+                out.gx[i] = acc[5] * 0.5;
+                out.gy[i] = acc[6] * 0.5;
+                out.gz[i] = acc[7] * 0.5;
+                out.vsmx[i] = acc[8];
+            }
         }
-        acc[8] = (double)vsmx;
-        slice_reduce<9>(acc, red, M, lane, 1 << 8);
-        // hdr_hydro.hpp:402-419
-        if (M.lead) {
-            out.ax[i] = acc[0] * 0.5;
-            out.ay[i] = acc[1] * 0.5;
-            out.az[i] = acc[2] * 0.5;
-            out.udot[i] = acc[3] * 0.5;
-            out.diffu[i] = acc[4] * 2.;
-            out.gx[i] = acc[5] * 0.5;
-            out.gy[i] = acc[6] * 0.5;
-            out.gz[i] = acc[7] * 0.5;
-            out.vsmx[i] = acc[8];
-        }
-    }
+    };
+    run_walks<PERSIST>(n_walk, queue, walk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1132,11 +1167,11 @@ __device__ __forceinline__ void gravity_list_rel(const Map &M, int lane, int kb,
 
 // ETA false: the caller does not need the softening-gradient sum (out.eta
 // NULL, e.g. under a constant softening, where it does not enter).
-template <bool MIXED, bool UNIFORM, bool ETA>
-__global__ __launch_bounds__(kWave) void gravity_kernel(
+template <bool MIXED, bool UNIFORM, bool ETA, bool PERSIST>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PERSIST && MIXED ? 8 : 1))) void gravity_kernel(
     int n_walk, const int32_t *__restrict__ i_off, const int32_t *__restrict__ j_off,
     const int32_t *__restrict__ j_idx, sphk_grav_src pj, const int32_t *__restrict__ s_off,
-    const int32_t *__restrict__ s_idx, sphk_grav_src ps, sphk_grav_src pi, sphk_grav_out out)
+    const int32_t *__restrict__ s_idx, sphk_grav_src ps, sphk_grav_src pi, sphk_grav_out out, int *queue)
 {
     constexpr int kTileBytes = kTile * sizeof(GRec<MIXED>);
     constexpr int kRelBytes = (MIXED && UNIFORM) ? kGravBlock * (int)sizeof(float4) : 0;
@@ -1146,62 +1181,91 @@ __global__ __launch_bounds__(kWave) void gravity_kernel(
     GRec<MIXED> *sp = reinterpret_cast<GRec<MIXED> *>(smem);
     double *red = reinterpret_cast<double *>(smem);
 
-    if ((int)blockIdx.x >= n_walk) return;
-    const int w = xcd_walk(blockIdx.x, n_walk);
-    const int lane = threadIdx.x;
-    const int ib = i_off[w], ie = i_off[w + 1];
+    auto walk = [&](const int w, const int lane) {
+        const int ib = i_off[w], ie = i_off[w + 1];
 
-    for (int c0 = ib; c0 < ie; c0 += kWave) {
-        const Map M(min(kWave, ie - c0), lane);
-        const int i = c0 + M.il;
-        const double px = pi.x[i], py = pi.y[i], pz = pi.z[i], e2 = pi.eps2[i];
-        double acc[5] = {0., 0., 0., 0., 0.};
-        if (MIXED && UNIFORM) {
-            const double cx = pi.x[ib], cy = pi.y[ib], cz = pi.z[ib];
-            const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
-            float4 *sr = reinterpret_cast<float4 *>(smem);
-            gravity_list_rel<ETA>(M, lane, j_off[w], j_off[w + 1], j_idx, pj, cx, cy, cz, xi, yi, zi,
-                                  (float)e2, sr, acc);
-            if (s_off)
-                gravity_list_rel<ETA>(M, lane, s_off[w], s_off[w + 1], s_idx, ps, cx, cy, cz, xi, yi, zi,
+        for (int c0 = ib; c0 < ie; c0 += kWave) {
+            const Map M(min(kWave, ie - c0), lane);
+            const int i = c0 + M.il;
+            const double px = pi.x[i], py = pi.y[i], pz = pi.z[i], e2 = pi.eps2[i];
+            double acc[5] = {0., 0., 0., 0., 0.};
+            if (MIXED && UNIFORM) {
+                const double cx = pi.x[ib], cy = pi.y[ib], cz = pi.z[ib];
+                const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
+                float4 *sr = reinterpret_cast<float4 *>(smem);
+                gravity_list_rel<ETA>(M, lane, j_off[w], j_off[w + 1], j_idx, pj, cx, cy, cz, xi, yi, zi,
                                       (float)e2, sr, acc);
-        } else {
-            gravity_list<MIXED, UNIFORM, ETA>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
-                                              acc);
-            if (s_off)
-                gravity_list<MIXED, UNIFORM, ETA>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
-                                                  sp, acc);
-        }
-        slice_reduce<5>(acc, red, M, lane);
-        if (M.lead) {
-            if (UNIFORM) {
-                // 0.5 * (-2 sum m rei3 x) etc.
-                out.ax[i] = -acc[0];
-                out.ay[i] = -acc[1];
-                out.az[i] = -acc[2];
-                out.pot[i] = -acc[3];
+                if (s_off)
+                    gravity_list_rel<ETA>(M, lane, s_off[w], s_off[w + 1], s_idx, ps, cx, cy, cz, xi, yi, zi,
+                                          (float)e2, sr, acc);
             } else {
-                out.ax[i] = acc[0] * 0.5;
-                out.ay[i] = acc[1] * 0.5;
-                out.az[i] = acc[2] * 0.5;
-                out.pot[i] = acc[3] * 0.5;
+                gravity_list<MIXED, UNIFORM, ETA>(lane, M, j_off[w], j_off[w + 1], j_idx, pj, px, py, pz, e2, sp,
+                                                  acc);
+                if (s_off)
+                    gravity_list<MIXED, UNIFORM, ETA>(lane, M, s_off[w], s_off[w + 1], s_idx, ps, px, py, pz, e2,
+                                                      sp, acc);
             }
-            if (ETA) out.eta[i] = acc[4];
+            slice_reduce<5>(acc, red, M, lane);
+            if (M.lead) {
+                if (UNIFORM) {
+                    // 0.5 * (-2 sum m rei3 x) etc.
+                    out.ax[i] = -acc[0];
+                    out.ay[i] = -acc[1];
+                    out.az[i] = -acc[2];
+                    out.pot[i] = -acc[3];
+                } else {
+                    out.ax[i] = acc[0] * 0.5;
+                    out.ay[i] = acc[1] * 0.5;
+                    out.az[i] = acc[2] * 0.5;
+                    out.pot[i] = acc[3] * 0.5;
+                }
+                if (ETA) out.eta[i] = acc[4];
+            }
         }
-    }
+    };
+    run_walks<PERSIST>(n_walk, queue, walk);
 }
 
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+// Grid of a persistent launch: the workgroups of this kernel the chip holds
+// at once (a multiple of kXcd), at most one per walk of each XCD queue.
+template <typename K>
+static int persistent_grid(K kernel, int n_walk)
+{
+    int dev = 0, n_cu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0) != hipSuccess)
+        return -1;
+    const int full = (per_cu > 0 ? per_cu : 1) * (n_cu > 0 ? n_cu : 1);
+    const int g = (full + kXcd - 1) / kXcd * kXcd;
+    const int need = (n_walk + kXcd - 1) / kXcd * kXcd;
+    return g < need ? g : need;
+}
+
+// One launch: ordinary (grid = n_walk) or, with a queue, persistent.
+#define SPHK_LAUNCH(KERNEL, KERNEL_P, ...)                                                     \
+    do {                                                                                        \
+        if (queue) {                                                                            \
+            const int g_ = persistent_grid(KERNEL_P, n_walk);                                   \
+            if (g_ <= 0) return hipErrorInvalidValue;                                           \
+            hipLaunchKernelGGL(KERNEL_P, dim3(g_), dim3(kWave), 0, st, __VA_ARGS__, queue);     \
+        } else {                                                                                \
+            hipLaunchKernelGGL(KERNEL, dim3(n_walk), dim3(kWave), 0, st, __VA_ARGS__, nullptr); \
+        }                                                                                       \
+    } while (0)
+
 hipError_t launch_density(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_dens_src &pi, const sphk_dens_src &pj,
-                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap)
+                          const sphk_dens_out &out, double eta_ksrh, double hmax, bool mixed, int list_cap,
+                          int *queue)
 {
     if (n_walk <= 0) return hipSuccess;
-#define SPHK_LAUNCH_D(M, CAPV)                                                                                     \
-    hipLaunchKernelGGL((density_kernel<M, CAPV>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, j_idx, \
-                       pi, pj, out, eta_ksrh, hmax)
+#define SPHK_LAUNCH_D(M, CAPV)                                                                           \
+    SPHK_LAUNCH((density_kernel<M, CAPV, false>), (density_kernel<M, CAPV, true>), n_walk, i_off, j_off, \
+                j_idx, pi, pj, out, eta_ksrh, hmax)
     if (list_cap == 192) {
         if (mixed)
             SPHK_LAUNCH_D(true, 192);
@@ -1219,27 +1283,27 @@ hipError_t launch_density(hipStream_t st, int n_walk, const int32_t *i_off, cons
 
 hipError_t launch_hydro(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                         const int32_t *j_idx, const sphk_hydro_src &pi, const sphk_hydro_src &pj,
-                        const sphk_hydro_out &out, bool mixed)
+                        const sphk_hydro_out &out, bool mixed, int *queue)
 {
     if (n_walk <= 0) return hipSuccess;
     if (mixed)
-        hipLaunchKernelGGL(hydro_kernel<true>, dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,
-                           j_off, j_idx, pi, pj, out);
+        SPHK_LAUNCH((hydro_kernel<true, false>), (hydro_kernel<true, true>), n_walk, i_off, j_off, j_idx, pi, pj,
+                    out);
     else
-        hipLaunchKernelGGL(hydro_kernel<false>, dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off,
-                           j_off, j_idx, pi, pj, out);
+        SPHK_LAUNCH((hydro_kernel<false, false>), (hydro_kernel<false, true>), n_walk, i_off, j_off, j_idx, pi,
+                    pj, out);
     return hipGetLastError();
 }
 
 hipError_t launch_gravity(hipStream_t st, int n_walk, const int32_t *i_off, const int32_t *j_off,
                           const int32_t *j_idx, const sphk_grav_src &pj, const int32_t *s_off,
                           const int32_t *s_idx, const sphk_grav_src &ps, const sphk_grav_src &pi,
-                          const sphk_grav_out &out, bool mixed, bool uniform)
+                          const sphk_grav_out &out, bool mixed, bool uniform, int *queue)
 {
     if (n_walk <= 0) return hipSuccess;
-#define SPHK_LAUNCH_G(M, U, E)                                                                         \
-    hipLaunchKernelGGL((gravity_kernel<M, U, E>), dim3(n_walk), dim3(kWave), 0, st, n_walk, i_off, j_off, \
-                       j_idx, pj, s_off, s_idx, ps, pi, out)
+#define SPHK_LAUNCH_G(M, U, E)                                                                   \
+    SPHK_LAUNCH((gravity_kernel<M, U, E, false>), (gravity_kernel<M, U, E, true>), n_walk, i_off, \
+                j_off, j_idx, pj, s_off, s_idx, ps, pi, out)
     const bool eta = out.eta != nullptr;
     if (mixed && uniform) {
         if (eta) SPHK_LAUNCH_G(true, true, true);
@@ -1257,5 +1321,6 @@ hipError_t launch_gravity(hipStream_t st, int n_walk, const int32_t *i_off, cons
 #undef SPHK_LAUNCH_G
     return hipGetLastError();
 }
+#undef SPHK_LAUNCH
 
 }  // namespace sphk

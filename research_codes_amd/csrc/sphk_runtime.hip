@@ -133,10 +133,17 @@ struct TimeRec {
 
 }  // namespace sphk
 
+// Walk queues of persistent launches: each launch takes the next of
+// kQueueSlots zeroed queues (the launch leaves it zeroed), so up to that many
+// persistent launches of one context may be in flight at once.
+constexpr int kQueueSlots = 64;
+
 struct sphk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    sphk_params params{1.6, 1.936492, 1e30, 0, 0};
+    sphk_params params{1.6, 1.936492, 1e30, 0, 0, 0};
+    int *queue = nullptr;          // kQueueSlots * kQueueInts ints on the device
+    unsigned queue_next = 0;
     sphk::TagSlot tags[SPHK_TAG_MAX];
     bool timing = false;
     std::vector<sphk::TimeRec> pending;
@@ -338,6 +345,16 @@ int sphk_init(int my_proc, sphk_ctx **out)
         delete ctx;
         return set_error(SPHK_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    const size_t qbytes = (size_t)kQueueSlots * kQueueInts * sizeof(int);
+    e = hipMalloc(&ctx->queue, qbytes);
+    if (e == hipSuccess) e = hipMemset(ctx->queue, 0, qbytes);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        if (ctx->queue) (void)hipFree(ctx->queue);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_error(SPHK_E_DEVICE, "walk queues: %s", hipGetErrorString(e));
+    }
     if (getenv("SPHK_VERBOSE")) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
@@ -363,6 +380,7 @@ int sphk_finalize(sphk_ctx *ctx)
         t.hin.release();
         t.hout.release();
     }
+    if (ctx->queue) (void)hipFree(ctx->queue);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SPHK_OK;
@@ -378,6 +396,8 @@ int sphk_set_params(sphk_ctx *ctx, const sphk_params *p)
                          p->ksrh, p->hmax);
     if (p->list_cap != 0 && p->list_cap != 192 && p->list_cap != 256)
         return set_error(SPHK_E_ARG, "list_cap must be 0, 192 or 256 (got %d)", p->list_cap);
+    if (p->persistent != 0 && p->persistent != 1)
+        return set_error(SPHK_E_ARG, "persistent must be 0 or 1 (got %d)", p->persistent);
     ctx->params = *p;
     return SPHK_OK;
 }
@@ -402,6 +422,17 @@ int sphk_synchronize(sphk_ctx *ctx)
 
 // ---- device-resident ----------------------------------------------------
 
+namespace {
+// The walk queue of the next launch: nullptr for one workgroup per walk.
+int *walk_queue(sphk_ctx *ctx)
+{
+    if (!ctx->params.persistent) return nullptr;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const unsigned slot = ctx->queue_next++ % kQueueSlots;
+    return ctx->queue + (size_t)slot * kQueueInts;
+}
+}  // namespace
+
 int sphk_density_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_t *j_off,
                      const int32_t *j_idx, sphk_dens_src pi, sphk_dens_src pj, sphk_dens_out out,
                      int precision, void *stream)
@@ -420,7 +451,8 @@ int sphk_density_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
     Timer t(ctx, 0, st);
     SPHK_HIP(launch_density(st, n_walk, i_off, j_off, j_idx, pi, pj, out,
                             ctx->params.eta * ctx->params.ksrh, ctx->params.hmax,
-                            precision == SPHK_MIXED, ctx->params.list_cap == 192 ? 192 : 256));
+                            precision == SPHK_MIXED, ctx->params.list_cap == 192 ? 192 : 256,
+                            walk_queue(ctx)));
     t.stop();
     return SPHK_OK;
 }
@@ -446,7 +478,8 @@ int sphk_hydro_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int32_
         if (!fo[k]) return set_error(SPHK_E_ARG, "hydro: null output field %d", k);
     hipStream_t st = pick_stream(ctx, stream);
     Timer t(ctx, 1, st);
-    SPHK_HIP(launch_hydro(st, n_walk, i_off, j_off, j_idx, pi, pj, out, precision == SPHK_MIXED));
+    SPHK_HIP(launch_hydro(st, n_walk, i_off, j_off, j_idx, pi, pj, out, precision == SPHK_MIXED,
+                          walk_queue(ctx)));
     t.stop();
     return SPHK_OK;
 }
@@ -471,7 +504,7 @@ int sphk_gravity_dev(sphk_ctx *ctx, int n_walk, const int32_t *i_off, const int3
     hipStream_t st = pick_stream(ctx, stream);
     Timer t(ctx, 2, st);
     SPHK_HIP(launch_gravity(st, n_walk, i_off, j_off, j_idx, pj, s_off, s_idx, ps, pi, out,
-                            precision == SPHK_MIXED, ctx->params.uniform_softening != 0));
+                            precision == SPHK_MIXED, ctx->params.uniform_softening != 0, walk_queue(ctx)));
     t.stop();
     return SPHK_OK;
 }

@@ -135,7 +135,8 @@ class Context:
         except Exception:
             pass
 
-    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None, list_cap=None):
+    def set_params(self, eta=None, ksrh=None, hmax=None, uniform_softening=None, list_cap=None,
+                   persistent=None):
         p = L.Params()
         if getattr(self, "_params", None) is not None:
             p = self._params
@@ -149,13 +150,16 @@ class Context:
             p.uniform_softening = int(bool(uniform_softening))
         if list_cap is not None:
             p.list_cap = int(list_cap)
+        if persistent is not None:
+            p.persistent = int(bool(persistent))
         check(lib().sphk_set_params(self._h, C.byref(p)), "sphk_set_params")
         self._params = p
 
     @property
     def params(self):
         return dict(eta=self._params.eta, ksrh=self._params.ksrh, hmax=self._params.hmax,
-                    uniform_softening=bool(self._params.uniform_softening), list_cap=self._params.list_cap)
+                    uniform_softening=bool(self._params.uniform_softening), list_cap=self._params.list_cap,
+                    persistent=bool(self._params.persistent))
 
     @staticmethod
     def _stream():

@@ -254,3 +254,6 @@ def test_list_cap_parameter_is_validated(ctx):
     with pytest.raises(D.L.SphkError):
         ctx.set_params(list_cap=100)
     ctx.set_params(list_cap=0)
+    import ctypes
+    p = D.L.Params(1.6, 1.936492, 1e30, 0, 0, 2)      # persistent must be 0 or 1
+    assert D.L.lib().sphk_set_params(ctx._h, ctypes.byref(p)) != 0

@@ -309,3 +309,60 @@ def test_collision_loss_finite_difference_sensitivity():
     g2 = fd_gradient(f, [0.0, 0.0], h=5e-5)
     print(f"[sens] loss {base:.9e} grad(h) {g1} grad(h/2) {g2}")
     assert np.all(np.isfinite(g1)) and np.all(np.abs(g1 - g2) <= 1e-2 * np.abs(g1).max() + 1e-9)
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_persistent_launches_equal_ordinary(prec):
+    """Persistent launches (grid-resident workgroups taking walks from the
+    per-XCD queues) give bitwise the ordinary launches' results: an ensemble
+    of three runs in one launch per functor, again (each launch leaves its
+    queue zeroed for the next), and replayed from a HIP graph; a run with
+    fewer walks than the grid holds workgroups (walk count not a multiple
+    of 8) too: the grid's surplus workgroups claim nothing."""
+    p, counts, _ = ic.ensemble(3, 1500, 3000, seed=78)
+    with Context(0) as c:
+        s = SPHSystem(p, c, eta=1.18, eps=0.01, precision=prec, n_group=16, hmax=1e30, segments=counts)
+        s.initialize()
+        s.rs.copy_(1.1 * s.s["ksr"])
+        s.build_lists()
+        snap = {k: v.clone() for k, v in s.s.items()}
+        keys = ("dens", "ksr", "np", "ax", "ay", "az", "udot", "pot", "vsmx")
+
+        def step(persistent):
+            c.set_params(persistent=persistent)
+            for k, v in snap.items():
+                s.s[k].copy_(v)
+            s.force(prec)
+            torch.cuda.synchronize()
+            return {k: (s.np_ if k == "np" else s.s[k]).clone() for k in keys}
+
+        ref = step(False)
+        assert all(bool(torch.isfinite(v.double()).all()) for v in ref.values())
+        for rep in range(3):
+            got = step(True)
+            for k in keys:
+                assert torch.equal(got[k], ref[k]), (rep, k)
+        g = s.capture_force(prec)            # persistent is still set
+        for rep in range(3):
+            for k, v in snap.items():
+                s.s[k].copy_(v)
+            _ = (s.s["x"] * 2.0).sum().item()
+            g.replay()
+            torch.cuda.synchronize()
+            for k in keys:
+                assert torch.equal((s.np_ if k == "np" else s.s[k]), ref[k]), ("graph", rep, k)
+        c.set_params(persistent=False)
+
+    small = ic.head_on(200, eta=1.18)
+    out = []
+    for persistent in (False, True):
+        with Context(0) as c:
+            c.set_params(persistent=persistent)
+            s = SPHSystem(small, c, eta=1.18, eps=0.01, precision=prec, n_group=16)
+            s.initialize()
+            print(f"[persistent] small run: walks {s.lists['dens'].n_walk} / {s.lists['hydro'].n_walk} / "
+                  f"{s.lists['grav'].n_walk}", flush=True)
+            out.append({k: s.s[k].clone() for k in ("dens", "ax", "ay", "az", "udot")})
+    assert all(bool(torch.isfinite(v).all()) for v in out[0].values())
+    for k in out[0]:
+        assert torch.equal(out[0][k], out[1][k]), k
