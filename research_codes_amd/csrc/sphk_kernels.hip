@@ -787,11 +787,12 @@ struct HydroAcc {
 
 template <typename R>
 __device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const R4<R> &A, const R4<R> &B,
-                                           const R4<R> &Cc, R eta_j, int32_t jid,
+                                           const R4<R> &Cc, R eh_j, R h5_j, int32_t jid,
                                            const HydroI<R> &I, HydroAcc<R> &a)
 {
-    // p = {x, y, z, m}; A = {vx, vy, vz, hinv}; B = {pres, presc, bswt, dens};
-    // Cc = {vsnd, alph, alphu, thrm}
+    // p = {x, y, z, -5 C1 m}; A = {vx, vy, vz, hinv}; B = {pres, presc, bswt, dens};
+    // Cc = {vsnd, alph, alphu, thrm}; eh_j = eta_j h_j^5, h5_j = h_j^5 (formed
+    // per record when it is staged)
     const R dpx = (R)I.px - (R)p.x;
     const R dpy = (R)I.py - (R)p.y;
     const R dpz = (R)I.pz - (R)p.z;
@@ -812,11 +813,10 @@ __device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const
     R ti = R(1) - q_i, tj = R(1) - q_j;
     ti = ti > R(0) ? ti : R(0);
     tj = tj > R(0) ? tj : R(0);
-    const R hj2 = hj * hj;
+    const R tj3 = tj * tj * tj;
     const R Fi = I.h5 * (ti * ti * ti);
-    const R Fj = (hj2 * hj2 * hj) * (tj * tj * tj);
-    const R m_j = (R)p.w;
-    const R mk = other ? R(-5. * kCeff1) * m_j : R(0);
+    const R Fj = h5_j * tj3;
+    const R mk = other ? (R)p.w : R(0);
     const R kri = mk * (Fi + Fj);   // ka * ri, ka = (dW_i + dW_j) m_j
     const R ka = kri * r1;
 
@@ -839,7 +839,7 @@ __device__ __forceinline__ void hydro_pair(const typename HPos<R>::rec &p, const
     a.ud += ka * (I.pc * w_ij - rhi * (R(0.25) * av0 * w0 - (I.au + Cc.c) * vsu * du));
     a.df += rhi * du * kri;
 
-    const R dg = mk * (I.et * Fi + eta_j * Fj);   // m_j ri (eta_i dW_i + eta_j dW_j)
+    const R dg = mk * (I.et * Fi + eh_j * tj3);   // m_j ri (eta_i dW_i + eta_j dW_j)
     a.gx += dg * dpx;
     a.gy += dg * dpy;
     a.gz += dg * dpz;
@@ -851,12 +851,14 @@ template <typename R>
 struct HRec {
     typename HPos<R>::rec p;
     R4<R> a, b, c;
-    R e;
+    R e, h5;
     int32_t id;
 };
 
 // Record k of the list, or an inert record for k >= end (beyond every
-// support, zero mass, signal speed -1e20 so it never sets vsmx).
+// support, zero mass, signal speed -1e20 so it never sets vsmx).  The
+// per-record factors of the pair term are formed here in FP64: -5 C1 m_j,
+// h_j^5 = hinv_j^5 and eta_j h_j^5.
 template <typename R>
 __device__ __forceinline__ HRec<R> hydro_load(const sphk_hydro_src &pj, const int32_t *__restrict__ j_idx, int k,
                                               int end, double cx, double cy, double cz)
@@ -864,13 +866,16 @@ __device__ __forceinline__ HRec<R> hydro_load(const sphk_hydro_src &pj, const in
     using PR = typename HPos<R>::rec;
     if (k < end) {
         const int j = j_idx ? j_idx[k] : k;
-        return HRec<R>{PR{(R)(pj.x[j] - cx), (R)(pj.y[j] - cy), (R)(pj.z[j] - cz), (R)pj.m[j]},
-                       R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)pj.hinv[j]},
+        const double hid = pj.hinv[j];
+        const double h5 = hid * (hid * hid * hid) * hid;
+        return HRec<R>{PR{(R)(pj.x[j] - cx), (R)(pj.y[j] - cy), (R)(pj.z[j] - cz), (R)(-5. * kCeff1 * pj.m[j])},
+                       R4<R>{(R)pj.vx[j], (R)pj.vy[j], (R)pj.vz[j], (R)hid},
                        R4<R>{(R)pj.pres[j], (R)pj.presc[j], (R)pj.bswt[j], (R)pj.dens[j]},
-                       R4<R>{(R)pj.vsnd[j], (R)pj.alph[j], (R)pj.alphu[j], (R)pj.thrm[j]}, (R)pj.eta[j], pj.id[j]};
+                       R4<R>{(R)pj.vsnd[j], (R)pj.alph[j], (R)pj.alphu[j], (R)pj.thrm[j]}, (R)(pj.eta[j] * h5),
+                       (R)h5, pj.id[j]};
     }
     return HRec<R>{PR{(R)kFar, (R)kFar, (R)kFar, R(0)}, R4<R>{R(0), R(0), R(0), R(0)},
-                   R4<R>{R(0), R(0), R(0), R(1)}, R4<R>{R(-1e20), R(0), R(0), R(0)}, R(0), -1};
+                   R4<R>{R(0), R(0), R(0), R(1)}, R4<R>{R(-1e20), R(0), R(0), R(0)}, R(0), R(0), -1};
 }
 
 // FP64: at most 168 VGPRs (3 waves per SIMD, a few spills): 11 % faster than
@@ -884,7 +889,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? (
     // tile records and the slice-reduction buffer share one LDS block (the
     // reduction runs after the last tile)
     using PR = typename HPos<R>::rec;
-    constexpr int kTileBytes = kTile * (sizeof(PR) + 3 * sizeof(R4<R>) + sizeof(R) + 4);
+    constexpr int kTileBytes = kTile * (sizeof(PR) + 3 * sizeof(R4<R>) + 2 * sizeof(R) + 4);
     constexpr int kRedBytes = 9 * kWave * sizeof(double);
     __shared__ __attribute__((aligned(32))) char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
     PR *sp = reinterpret_cast<PR *>(smem);
@@ -892,7 +897,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? (
     R4<R> *sb = sa + kTile;
     R4<R> *sc = sb + kTile;
     R *se = reinterpret_cast<R *>(sc + kTile);
-    int32_t *sid = reinterpret_cast<int32_t *>(se + kTile);
+    R *sh = se + kTile;
+    int32_t *sid = reinterpret_cast<int32_t *>(sh + kTile);
     double *red = reinterpret_cast<double *>(smem);
 
     auto walk = [&](const int w, const int lane) {
@@ -939,6 +945,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? (
                 sb[lane] = nx.b;
                 sc[lane] = nx.c;
                 se[lane] = nx.e;
+                sh[lane] = nx.h5;
                 sid[lane] = nx.id;
                 __syncthreads();
                 HydroAcc<R> a{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), vsmx};
@@ -947,12 +954,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MIXED ? (
                     const int k1 = k + M.S;
                     const PR p0 = sp[k], p1 = sp[k1];
                     const R4<R> a0 = sa[k], a1 = sa[k1], b0 = sb[k], b1 = sb[k1], c0_ = sc[k], c1 = sc[k1];
-                    const R e0 = se[k], e1 = se[k1];
+                    const R e0 = se[k], e1 = se[k1], f0 = sh[k], f1 = sh[k1];
                     const int32_t d0 = sid[k], d1 = sid[k1];
-                    hydro_pair<R>(p0, a0, b0, c0_, e0, d0, I, a);
-                    hydro_pair<R>(p1, a1, b1, c1, e1, d1, I, a);
+                    hydro_pair<R>(p0, a0, b0, c0_, e0, f0, d0, I, a);
+                    hydro_pair<R>(p1, a1, b1, c1, e1, f1, d1, I, a);
                 }
-                for (; k < nt; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sid[k], I, a);
+                for (; k < nt; k += M.S) hydro_pair<R>(sp[k], sa[k], sb[k], sc[k], se[k], sh[k], sid[k], I, a);
                 acc[0] += (double)a.ax;
                 acc[1] += (double)a.ay;
                 acc[2] += (double)a.az;
