@@ -676,18 +676,22 @@ __device__ __forceinline__ bool member(const TreeView &t, const double *__restri
 
 // The particles of the leaves reached into jout (MODE 0 gather, 1 symmetry:
 // the candidates of the short-range lists), and for MODE 2 (long) the SPJ
-// nodes into sout: with FILL from the walk's offsets oj/os, else the walk's
-// counts go to cj/cs.  err is set if a stack of STACK entries would
-// overflow.
-template <int MODE, bool FILL, int STACK>
+// nodes into sout.  FILL 0: the walk's counts go to cj/cs.  FILL 1: written
+// from the walk's offsets oj/os.  FILL 2 (short lists): written into a room
+// of `room` entries per walk (walk g from g * room), the count to cj and the
+// room's offset to cs; bit 2 of err is set if a walk does not fit its room
+// (its entries past the room are not written, its count is the room's).  Bit 1 of err is set if a
+// stack of STACK entries would overflow.
+template <int MODE, int FILL, int STACK>
 __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const double *__restrict__ wb,
                                                        const double *__restrict__ r, double inv_t2,
                                                        int64_t *__restrict__ cj, int64_t *__restrict__ cs,
                                                        const int64_t *__restrict__ oj, const int64_t *__restrict__ os,
                                                        int32_t *__restrict__ jout, int32_t *__restrict__ sout,
-                                                       int *__restrict__ err)
+                                                       int *__restrict__ err, int64_t room = 0)
 {
 #pragma clang fp contract(off)
+    static_assert(FILL != 2 || MODE != 2, "rooms are for the short-range lists");
     __shared__ uint2 stk[kListWalks][STACK];
     const int lane = threadIdx.x, sub = lane >> 3, c = lane & 7;
     const int g = blockIdx.x * kListWalks + sub;
@@ -706,10 +710,11 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
             glo[d] = t.mo.lo[d * C + gn];
             ghi[d] = t.mo.hi[d * C + gn];
         }
-        if (FILL) {
+        if (FILL == 1) {
             j0 = oj[g];
             if (MODE == 2) s0 = os[g];
         }
+        if (FILL == 2) j0 = (int64_t)g * room;
     }
     // emit the entries (w0, w1) of the lanes with em set, in lane order
     auto emit = [&](bool em, uint32_t w0, uint32_t w1) {
@@ -717,15 +722,20 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
         if (MODE == 2) {
             const bool is_s = em && ty == kTySpj;
             const unsigned sb = walk_bits(is_s, lane);
-            if (FILL && is_s) sout[s0 + ns + __popc(sb & below)] = (int32_t)w0;
+            if (FILL == 1 && is_s) sout[s0 + ns + __popc(sb & below)] = (int32_t)w0;
             ns += __popc(sb);
         }
         const bool is_l = em && ty == kTyLeaf;
         const int sz = is_l ? (int)(w1 & kCountMask) : 0;
         const int incl = walk_scan_incl(sz, c);
-        if (FILL && is_l) {
+        if (FILL == 1 && is_l) {
             const int64_t p0 = j0 + nj + incl - sz;
             for (int q = 0; q < sz; q++) jout[p0 + q] = (int32_t)w0 + q;
+        }
+        if (FILL == 2 && is_l) {
+            const int64_t p0 = nj + incl - sz;   // within the room
+            for (int q = 0; q < sz; q++)
+                if (p0 + q < room) jout[j0 + p0 + q] = (int32_t)w0 + q;
         }
         nj += __shfl(incl, 7, 8);
     };
@@ -779,9 +789,14 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
         }
     }
     if (bad && c == 0) atomicOr(err, 1);
-    if (!FILL && live && c == 0) {
+    if (FILL == 0 && live && c == 0) {
         cj[g] = nj;
         if (MODE == 2) cs[g] = ns;
+    }
+    if (FILL == 2 && live && c == 0) {
+        cj[g] = nj < room ? nj : room;   // what the room holds (all of it unless err bit 2)
+        cs[g] = j0;
+        if (nj > room) atomicOr(err, 2);
     }
 }
 
@@ -856,6 +871,10 @@ __global__ void offsets_kernel(int ng, const int64_t *__restrict__ scan, const i
 
 }  // namespace
 
+constexpr int64_t kRoomStart = 2048;          // candidates per walk
+constexpr int64_t kRoomMax = 1 << 16;
+constexpr int64_t kRoomMaxEntries = 1ll << 30;  // scratch entries (4 GiB)
+
 struct sphk_dtree {
     int n = 0, n_leaf = 8, n_group = 16;
     hipStream_t st = nullptr;
@@ -876,6 +895,9 @@ struct sphk_dtree {
     DArr<int> err;
     DArr<char> scratch;
     DLists lists[3];
+    // per-walk room of the one-walk short-range list builds (gather, symmetry),
+    // doubled after a build that overflowed it
+    int64_t room[2] = {kRoomStart, kRoomStart};
 
     NodeArrays nodes() { return NodeArrays{begin.p, end.p, child.p, nchild.p, level.p, parent.p, len.p}; }
     Moments moments()
@@ -994,16 +1016,48 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
                t->i_off.p};
     const double *wb = MODE == 2 ? nullptr : t->wbox.p;
     const dim3 lg((unsigned)((ng + kListWalks - 1) / kListWalks)), lb(64);
+    if constexpr (MODE != 2) {
+        // one walk into a room per walk: no count walk, one host read.  If a
+        // walk overflows its room or the small stack, the counted build below
+        // runs instead (and later builds get twice the room).
+        int64_t &room = t->room[MODE];
+        if ((int64_t)ng * room <= kRoomMaxEntries) {
+            SPHK_TRY(t->mscr.reserve((size_t)ng * (size_t)room, false, t->st));
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 2, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+                               t->i64a.p, t->i64c.p, nullptr, nullptr, t->mscr.p, nullptr, t->err.p, room);
+            hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, wb, r, t->i64c.p,
+                               t->i64a.p, t->mscr.p, t->i64b.p);
+            SPHK_TRY(hipGetLastError());
+            if (int rc = walk_offsets(t, ng, t->i64b.p, t->i64d.p, L.j_off.p, nullptr, nullptr, nullptr)) return rc;
+            int64_t tot1[2] = {0, 0};
+            int flags = 0;
+            if (int rc = read_totals(t, tot1, flags)) return rc;
+            if (flags == 0) {
+                if (tot1[0] > std::numeric_limits<int32_t>::max())
+                    return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
+                SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot1[0], 1), false, t->st));
+                hipLaunchKernelGGL(member_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->i64c.p, L.j_off.p,
+                                   t->mscr.p, L.j_idx.p);
+                SPHK_TRY(hipGetLastError());
+                L.n_j = tot1[0];
+                L.n_s = 0;
+                L.built = true;
+                return SPHK_OK;
+            }
+            if (flags & 2) room = std::min<int64_t>(room * 2, kRoomMax);
+            SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
+        }
+    }
     // count walk with the small stack, and with the worst-case one if a walk overflowed it
     int64_t tot[2] = {0, 0};
     int overflow = 0;
     bool big = false;
     for (;;) {
         if (big)
-            hipLaunchKernelGGL((walk_list_kernel<MODE, false, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 0, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
                                t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
         else
-            hipLaunchKernelGGL((walk_list_kernel<MODE, false, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 0, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
                                inv_t2, t->i64a.p, t->i64b.p, nullptr, nullptr, nullptr, nullptr, t->err.p);
         SPHK_TRY(hipGetLastError());
         if (int rc = walk_offsets(t, ng, t->i64a.p, t->i64c.p, L.j_off.p, MODE == 2 ? t->i64b.p : nullptr,
@@ -1022,10 +1076,10 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
         SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
         SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot[1], 1), false, t->st));
         if (big)
-            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 1, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
                                nullptr, nullptr, t->i64c.p, t->i64d.p, L.j_idx.p, L.s_idx.p, t->err.p);
         else
-            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 1, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
                                inv_t2, nullptr, nullptr, t->i64c.p, t->i64d.p, L.j_idx.p, L.s_idx.p, t->err.p);
         SPHK_TRY(hipGetLastError());
     } else {
@@ -1033,10 +1087,10 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
         // their counts to i64b
         SPHK_TRY(t->mscr.reserve(std::max<int64_t>(tot[0], 1), false, t->st));
         if (big)
-            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 1, kListStack>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
                                nullptr, nullptr, t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
         else
-            hipLaunchKernelGGL((walk_list_kernel<MODE, true, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
+            hipLaunchKernelGGL((walk_list_kernel<MODE, 1, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
                                inv_t2, nullptr, nullptr, t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
         hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, wb, r, t->i64c.p, t->i64a.p,
                            t->mscr.p, t->i64b.p);

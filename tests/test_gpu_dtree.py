@@ -170,3 +170,28 @@ def test_device_resort_matches_host():
             s.resort()
             res.append((s.id.clone(), s.s["x"].clone()))
     assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
+
+
+def test_short_lists_beyond_the_walk_room():
+    """The short-range lists are built in one walk into a fixed room of
+    candidates per walk; a build whose walks overflow it runs the counted
+    build instead, and later builds get a larger room.  Wide search radii
+    and groups of 256 overflow the first room: every build, before and after
+    the room grows, must still equal the host's lists."""
+    p, _ = sort_particles(_cases()["star"])
+    ht = HostTree(p["x"], p["y"], p["z"], p["m"], None, 8, 256)
+    dt = DeviceTree(_d(p["x"]), _d(p["y"]), _d(p["z"]), _d(p["m"]), None, 8, 256)
+    try:
+        for fac in (4.0, 4.0, 1.1, 6.0):
+            rs = fac * p["ksr"]
+            hl, dl = ht.gather(rs), dt.gather(_d(rs))
+            n_cand_max = int(np.diff(hl[0]).max())
+            print(f"[dtree room] fac {fac}: longest gather list {n_cand_max}", flush=True)
+            for a, b in zip(hl, dl):
+                assert np.array_equal(a, b.cpu().numpy()), fac
+            hs, ds = ht.symmetry(rs), dt.symmetry(_d(rs))
+            for a, b in zip(hs, ds):
+                assert np.array_equal(a, b.cpu().numpy()), fac
+    finally:
+        ht.close()
+        dt.close()
