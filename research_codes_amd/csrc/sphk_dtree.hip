@@ -677,21 +677,22 @@ __device__ __forceinline__ bool member(const TreeView &t, const double *__restri
 // The particles of the leaves reached into jout (MODE 0 gather, 1 symmetry:
 // the candidates of the short-range lists), and for MODE 2 (long) the SPJ
 // nodes into sout.  FILL 0: the walk's counts go to cj/cs.  FILL 1: written
-// from the walk's offsets oj/os.  FILL 2 (short lists): written into a room
-// of `room` entries per walk (walk g from g * room), the count to cj and the
-// room's offset to cs; bit 2 of err is set if a walk does not fit its room
-// (its entries past the room are not written, its count is the room's).  Bit 1 of err is set if a
-// stack of STACK entries would overflow.
+// from the walk's offsets oj/os.  FILL 2: written into rooms of `room`
+// (jout) and `room_s` (sout) entries per walk, walk g's from g * room and
+// g * room_s; short lists: the count to cj and the room's offset to cs;
+// long list: the counts to cj and cs.  Bit 2 (4) of err is set if a walk
+// does not fit its jout (sout) room: entries past it are not written and
+// its count is the room's.
+// Bit 1 of err is set if a stack of STACK entries would overflow.
 template <int MODE, int FILL, int STACK>
 __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const double *__restrict__ wb,
                                                        const double *__restrict__ r, double inv_t2,
                                                        int64_t *__restrict__ cj, int64_t *__restrict__ cs,
                                                        const int64_t *__restrict__ oj, const int64_t *__restrict__ os,
                                                        int32_t *__restrict__ jout, int32_t *__restrict__ sout,
-                                                       int *__restrict__ err, int64_t room = 0)
+                                                       int *__restrict__ err, int64_t room = 0, int64_t room_s = 0)
 {
 #pragma clang fp contract(off)
-    static_assert(FILL != 2 || MODE != 2, "rooms are for the short-range lists");
     __shared__ uint2 stk[kListWalks][STACK];
     const int lane = threadIdx.x, sub = lane >> 3, c = lane & 7;
     const int g = blockIdx.x * kListWalks + sub;
@@ -714,7 +715,10 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
             j0 = oj[g];
             if (MODE == 2) s0 = os[g];
         }
-        if (FILL == 2) j0 = (int64_t)g * room;
+        if (FILL == 2) {
+            j0 = (int64_t)g * room;
+            if (MODE == 2) s0 = (int64_t)g * room_s;
+        }
     }
     // emit the entries (w0, w1) of the lanes with em set, in lane order
     auto emit = [&](bool em, uint32_t w0, uint32_t w1) {
@@ -723,6 +727,7 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
             const bool is_s = em && ty == kTySpj;
             const unsigned sb = walk_bits(is_s, lane);
             if (FILL == 1 && is_s) sout[s0 + ns + __popc(sb & below)] = (int32_t)w0;
+        if (FILL == 2 && is_s && ns + __popc(sb & below) < room_s) sout[s0 + ns + __popc(sb & below)] = (int32_t)w0;
             ns += __popc(sb);
         }
         const bool is_l = em && ty == kTyLeaf;
@@ -795,8 +800,12 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
     }
     if (FILL == 2 && live && c == 0) {
         cj[g] = nj < room ? nj : room;   // what the room holds (all of it unless err bit 2)
-        cs[g] = j0;
+        if (MODE == 2)
+            cs[g] = ns < room_s ? ns : room_s;
+        else
+            cs[g] = j0;
         if (nj > room) atomicOr(err, 2);
+        if (MODE == 2 && ns > room_s) atomicOr(err, 4);
     }
 }
 
@@ -856,6 +865,18 @@ __global__ __launch_bounds__(64) void member_copy_kernel(int ng, const int64_t *
     for (int k = threadIdx.x; k < n; k += 64) jout[b + k] = scr[s0 + k];
 }
 
+// The long list's entries from their rooms (walk g's from g * room) to the
+// walk offsets.
+__global__ __launch_bounds__(64) void room_copy_kernel(int ng, int64_t room, const int32_t *__restrict__ off,
+                                                       const int32_t *__restrict__ scr, int32_t *__restrict__ out)
+{
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int64_t s0 = (int64_t)g * room;
+    const int b = off[g], n = off[g + 1] - b;
+    for (int k = threadIdx.x; k < n; k += 64) out[b + k] = scr[s0 + k];
+}
+
 // Walk offsets from the scanned per-walk counts, and the total.
 __global__ void offsets_kernel(int ng, const int64_t *__restrict__ scan, const int64_t *__restrict__ cnt,
                                int32_t *__restrict__ off, int64_t *__restrict__ total)
@@ -871,9 +892,10 @@ __global__ void offsets_kernel(int ng, const int64_t *__restrict__ scan, const i
 
 }  // namespace
 
-constexpr int64_t kRoomStart = 2048;          // candidates per walk
+constexpr int64_t kRoomStart = 2048;          // candidates (SPJ nodes) per walk
+constexpr int64_t kRoomLongStart = 8192;      // long-list particles per walk
 constexpr int64_t kRoomMax = 1 << 16;
-constexpr int64_t kRoomMaxEntries = 1ll << 30;  // scratch entries (4 GiB)
+constexpr int64_t kRoomMaxEntries = 1ll << 31;  // scratch entries (8 GiB)
 
 struct sphk_dtree {
     int n = 0, n_leaf = 8, n_group = 16;
@@ -895,9 +917,12 @@ struct sphk_dtree {
     DArr<int> err;
     DArr<char> scratch;
     DLists lists[3];
-    // per-walk room of the one-walk short-range list builds (gather, symmetry),
-    // doubled after a build that overflowed it
+    // per-walk rooms of the one-walk list builds (gather and symmetry
+    // candidates; long-list EPJ and SPJ entries), doubled after a build that
+    // overflowed them
     int64_t room[2] = {kRoomStart, kRoomStart};
+    int64_t room_long[2] = {kRoomLongStart, kRoomStart};
+    DArr<int32_t> sscr;   // SPJ room of the long list
 
     NodeArrays nodes() { return NodeArrays{begin.p, end.p, child.p, nchild.p, level.p, parent.p, len.p}; }
     Moments moments()
@@ -1045,6 +1070,43 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
                 return SPHK_OK;
             }
             if (flags & 2) room = std::min<int64_t>(room * 2, kRoomMax);
+            SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
+        }
+    }
+    if constexpr (MODE == 2) {
+        // one walk into rooms per walk (EPJ and SPJ), then the entries packed
+        // to the walk offsets: no count walk, one host read.  The counted
+        // build below if a walk overflows a room or the small stack (and later
+        // builds get twice the room that overflowed).
+        int64_t &rj = t->room_long[0], &rsp = t->room_long[1];
+        if ((int64_t)ng * rj <= kRoomMaxEntries && (int64_t)ng * rsp <= kRoomMaxEntries) {
+            SPHK_TRY(t->mscr.reserve((size_t)ng * (size_t)rj, false, t->st));
+            SPHK_TRY(t->sscr.reserve((size_t)ng * (size_t)rsp, false, t->st));
+            hipLaunchKernelGGL((walk_list_kernel<2, 2, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
+                               t->i64a.p, t->i64b.p, nullptr, nullptr, t->mscr.p, t->sscr.p, t->err.p, rj, rsp);
+            SPHK_TRY(hipGetLastError());
+            if (int rc = walk_offsets(t, ng, t->i64a.p, t->i64c.p, L.j_off.p, t->i64b.p, t->i64d.p, L.s_off.p))
+                return rc;
+            int64_t tot1[2] = {0, 0};
+            int flags = 0;
+            if (int rc = read_totals(t, tot1, flags)) return rc;
+            if (flags == 0) {
+                if (tot1[0] > std::numeric_limits<int32_t>::max() || tot1[1] > std::numeric_limits<int32_t>::max())
+                    return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
+                SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot1[0], 1), false, t->st));
+                SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot1[1], 1), false, t->st));
+                hipLaunchKernelGGL(room_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, rj, L.j_off.p, t->mscr.p,
+                                   L.j_idx.p);
+                hipLaunchKernelGGL(room_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, rsp, L.s_off.p, t->sscr.p,
+                                   L.s_idx.p);
+                SPHK_TRY(hipGetLastError());
+                L.n_j = tot1[0];
+                L.n_s = tot1[1];
+                L.built = true;
+                return SPHK_OK;
+            }
+            if (flags & 2) rj = std::min<int64_t>(rj * 2, kRoomMax);
+            if (flags & 4) rsp = std::min<int64_t>(rsp * 2, kRoomMax);
             SPHK_TRY(hipMemsetAsync(t->err.p, 0, sizeof(int), t->st));
         }
     }

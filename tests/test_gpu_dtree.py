@@ -195,3 +195,25 @@ def test_short_lists_beyond_the_walk_room():
     finally:
         ht.close()
         dt.close()
+
+
+def test_long_lists_beyond_the_walk_room():
+    """The long list is built in one walk into rooms of particle (EPJ) and
+    node (SPJ) entries per walk, then packed.  Direct summation (theta 0) of
+    10 000 particles overflows the first EPJ room (8192): the counted build
+    runs instead and later builds get twice the room; every build must equal
+    the host's lists."""
+    p = ic.polytrope(1.5, 10000, M=1.0, R=1.0, seed=12, eta=1.6)
+    p, _ = sort_particles(p)
+    ht = HostTree(p["x"], p["y"], p["z"], p["m"], None, 8, 64)
+    dt = DeviceTree(_d(p["x"]), _d(p["y"]), _d(p["z"]), _d(p["m"]), None, 8, 64)
+    try:
+        for theta in (0.0, 0.0, 0.5, 0.3):
+            hl, dl = ht.long(theta), dt.long(theta)
+            print(f"[dtree room] theta {theta}: longest EPJ list {int(np.diff(hl[0]).max())}", flush=True)
+            assert len(hl) == len(dl)
+            for a, b in zip(hl, dl):
+                assert np.array_equal(a, b.cpu().numpy()), theta
+    finally:
+        ht.close()
+        dt.close()
