@@ -58,7 +58,14 @@ int sphk_dtree_n_node(const sphk_dtree *t);
 /* Walk table: n_walk + 1 offsets into the particle order (device). */
 const int32_t *sphk_dtree_i_off(const sphk_dtree *t);
 
-/* kind 0: gather (radius rs), 1: symmetry (radius h), 2: long (theta). */
+/* kind 0: gather (radius rs), 1: symmetry (radius h), 2: long (theta).
+ * Each build walks the tree once into a fixed room of entries per walk in
+ * the handle's scratch (2048 candidates per walk for kinds 0 and 1; 8192
+ * particles and 2048 nodes for kind 2) and packs the result; if a walk
+ * overflows its room (or the walk's small stack), the build runs again as a
+ * count walk, a scan and a fill walk, and the handle doubles that room for
+ * later builds.  The scratch (n_walk x room entries, at most 8 GiB) is kept
+ * with the handle's other buffers.  One host read per build either way. */
 int sphk_dtree_list_gather(sphk_dtree *t, const double *rs);
 int sphk_dtree_list_symmetry(sphk_dtree *t, const double *h);
 int sphk_dtree_list_long(sphk_dtree *t, double theta);
