@@ -69,10 +69,20 @@ const int32_t *sphk_dtree_i_off(const sphk_dtree *t);
 int sphk_dtree_list_gather(sphk_dtree *t, const double *rs);
 int sphk_dtree_list_symmetry(sphk_dtree *t, const double *h);
 int sphk_dtree_list_long(sphk_dtree *t, double theta);
-/* Device offsets and indices of a built list; n_j / n_s receive the totals
- * (s_* and n_s are NULL / 0 for kinds 0 and 1). */
-int sphk_dtree_list(const sphk_dtree *t, int kind, const int32_t **j_off, const int32_t **j_idx,
+/* Device offsets and indices of a built list, held by the handle; n_j / n_s
+ * receive the totals (s_* and n_s are NULL / 0 for kinds 0 and 1).  A list
+ * still in its rooms is packed into the handle's arrays first. */
+int sphk_dtree_list(sphk_dtree *t, int kind, const int32_t **j_off, const int32_t **j_idx,
                     const int32_t **s_off, const int32_t **s_idx, int64_t *n_j, int64_t *n_s);
+/* The totals of a built list (n_s 0 for kinds 0 and 1). */
+int sphk_dtree_list_size(const sphk_dtree *t, int kind, int64_t *n_j, int64_t *n_s);
+/* A built list into the caller's device arrays (j_off and s_off n_walk + 1
+ * entries, j_idx n_j, s_idx n_s; s_* only for kind 2), on the handle's
+ * stream: from the rooms straight to the caller's arrays when the list was
+ * built in one walk (one pass over the entries instead of a pack and a
+ * copy), else a copy of the handle's arrays. */
+int sphk_dtree_list_copy(sphk_dtree *t, int kind, int32_t *j_off, int32_t *j_idx, int32_t *s_off,
+                         int32_t *s_idx);
 /* Node monopoles (device, n_node entries each). */
 int sphk_dtree_moments(const sphk_dtree *t, const double **m, const double **x, const double **y,
                        const double **z, const double **eps2);

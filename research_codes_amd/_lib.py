@@ -138,6 +138,8 @@ SIGNATURES.update({
     "sphk_dtree_list_long": (C.c_int, [_P, C.c_double]),
     "sphk_dtree_list": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                   C.POINTER(_P), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "sphk_dtree_list_size": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "sphk_dtree_list_copy": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "sphk_dtree_moments": (C.c_int, [_P] + [C.POINTER(_P)] * 5),
     "sphk_dtree_nodes": (C.c_int, [_P] + [C.POINTER(_P)] * 6),
     "sphk_copy_dev": (C.c_int, [_P, _P, C.c_size_t, _P]),

@@ -85,6 +85,11 @@ struct DLists {
     DArr<int32_t> j_off, j_idx, s_off, s_idx;
     int64_t n_j = 0, n_s = 0;
     bool built = false;
+    // built by one walk and not yet packed: the entries are still in the
+    // rooms of the handle's scratch (walk g's from g * room_j in mscr, from
+    // g * room_s in sscr for the long list's nodes); j_off / s_off are final
+    bool in_rooms = false;
+    int64_t room_j = 0, room_s = 0;
 };
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v)
@@ -1012,6 +1017,32 @@ int walk_offsets(sphk_dtree *t, int ng, const int64_t *a, int64_t *c, int32_t *o
     return SPHK_OK;
 }
 
+// A list still in the rooms of the scratch: packed to its offsets, into the
+// caller's arrays (j_idx / s_idx non-NULL; the list stays in the rooms) or
+// into the handle's own (the list leaves the rooms).
+int pack_list(sphk_dtree *t, int kind, int32_t *j_idx, int32_t *s_idx)
+{
+    DLists &L = t->lists[kind];
+    if (!L.built || !L.in_rooms) return SPHK_OK;
+    const int ng = t->n_walk;
+    const bool own = j_idx == nullptr;
+    if (own) {
+        SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(L.n_j, 1), false, t->st));
+        j_idx = L.j_idx.p;
+        if (kind == 2) {
+            SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(L.n_s, 1), false, t->st));
+            s_idx = L.s_idx.p;
+        }
+    }
+    hipLaunchKernelGGL(room_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, L.room_j, L.j_off.p, t->mscr.p, j_idx);
+    if (kind == 2 && s_idx)
+        hipLaunchKernelGGL(room_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, L.room_s, L.s_off.p, t->sscr.p,
+                           s_idx);
+    SPHK_TRY(hipGetLastError());
+    if (own) L.in_rooms = false;
+    return SPHK_OK;
+}
+
 // Interaction lists of every walk.  Long list: the tree walk (count, scan,
 // fill) gives each walk's EPJ particles and SPJ nodes.  Short-range lists
 // (gather: r = rs, symmetry: r = H): the tree walk (count, scan, fill)
@@ -1021,8 +1052,12 @@ int walk_offsets(sphk_dtree *t, int ng, const int64_t *a, int64_t *c, int32_t *o
 template <int MODE>
 int build_list(sphk_dtree *t, const double *r, double inv_t2)
 {
+    // lists still in the rooms of the scratch this build reuses: packed first
+    for (int k = 0; k < 3; k++)
+        if (int rc = pack_list(t, k, nullptr, nullptr)) return rc;
     DLists &L = t->lists[MODE];
     L.built = false;
+    L.in_rooms = false;
     const int ng = t->n_walk;
     SPHK_TRY(t->i64a.reserve(ng, false, t->st));   // EPJ counts per walk
     SPHK_TRY(t->i64b.reserve(ng, false, t->st));   // SPJ counts
@@ -1060,12 +1095,12 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
             if (flags == 0) {
                 if (tot1[0] > std::numeric_limits<int32_t>::max())
                     return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
-                SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot1[0], 1), false, t->st));
-                hipLaunchKernelGGL(member_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, t->i64c.p, L.j_off.p,
-                                   t->mscr.p, L.j_idx.p);
-                SPHK_TRY(hipGetLastError());
+                // the members stay in their rooms until the list is read
                 L.n_j = tot1[0];
                 L.n_s = 0;
+                L.in_rooms = true;
+                L.room_j = room;
+                L.room_s = 0;
                 L.built = true;
                 return SPHK_OK;
             }
@@ -1093,15 +1128,12 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
             if (flags == 0) {
                 if (tot1[0] > std::numeric_limits<int32_t>::max() || tot1[1] > std::numeric_limits<int32_t>::max())
                     return set_error(SPHK_E_CAPACITY, "dtree: list longer than 2^31 entries");
-                SPHK_TRY(L.j_idx.reserve(std::max<int64_t>(tot1[0], 1), false, t->st));
-                SPHK_TRY(L.s_idx.reserve(std::max<int64_t>(tot1[1], 1), false, t->st));
-                hipLaunchKernelGGL(room_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, rj, L.j_off.p, t->mscr.p,
-                                   L.j_idx.p);
-                hipLaunchKernelGGL(room_copy_kernel, dim3(ng), dim3(64), 0, t->st, ng, rsp, L.s_off.p, t->sscr.p,
-                                   L.s_idx.p);
-                SPHK_TRY(hipGetLastError());
+                // the entries stay in their rooms until the list is read
                 L.n_j = tot1[0];
                 L.n_s = tot1[1];
+                L.in_rooms = true;
+                L.room_j = rj;
+                L.room_s = rsp;
                 L.built = true;
                 return SPHK_OK;
             }
@@ -1239,7 +1271,10 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
     t->st = stream;
     t->n_walk = 0;
     t->n_node = 0;
-    for (auto &L : t->lists) L.built = false;
+    for (auto &L : t->lists) {
+        L.built = false;
+        L.in_rooms = false;
+    }
     auto fail = [&](int rc) {
         t->n_walk = 0;
         t->n_node = 0;
@@ -1434,18 +1469,48 @@ int sphk_dtree_list_long(sphk_dtree *t, double theta)
     return build_list<2>(t, nullptr, inv_t2);
 }
 
-int sphk_dtree_list(const sphk_dtree *t, int kind, const int32_t **j_off, const int32_t **j_idx,
+int sphk_dtree_list(sphk_dtree *t, int kind, const int32_t **j_off, const int32_t **j_idx,
                     const int32_t **s_off, const int32_t **s_idx, int64_t *n_j, int64_t *n_s)
 {
     if (!t || kind < 0 || kind > 2) return set_error(SPHK_E_ARG, "dtree list: bad argument");
-    const DLists &L = t->lists[kind];
+    DLists &L = t->lists[kind];
     if (!L.built) return set_error(SPHK_E_STATE, "dtree list kind %d not built", kind);
+    if (int rc = pack_list(t, kind, nullptr, nullptr)) return rc;
     if (j_off) *j_off = L.j_off.p;
     if (j_idx) *j_idx = L.j_idx.p;
     if (s_off) *s_off = kind == 2 ? L.s_off.p : nullptr;
     if (s_idx) *s_idx = kind == 2 ? L.s_idx.p : nullptr;
     if (n_j) *n_j = L.n_j;
     if (n_s) *n_s = L.n_s;
+    return SPHK_OK;
+}
+
+int sphk_dtree_list_size(const sphk_dtree *t, int kind, int64_t *n_j, int64_t *n_s)
+{
+    if (!t || kind < 0 || kind > 2) return set_error(SPHK_E_ARG, "dtree list size: bad argument");
+    const DLists &L = t->lists[kind];
+    if (!L.built) return set_error(SPHK_E_STATE, "dtree list kind %d not built", kind);
+    if (n_j) *n_j = L.n_j;
+    if (n_s) *n_s = L.n_s;
+    return SPHK_OK;
+}
+
+int sphk_dtree_list_copy(sphk_dtree *t, int kind, int32_t *j_off, int32_t *j_idx, int32_t *s_off, int32_t *s_idx)
+{
+    if (!t || kind < 0 || kind > 2 || !j_off || (!j_idx && t->lists[kind].n_j > 0))
+        return set_error(SPHK_E_ARG, "dtree list copy: bad argument");
+    DLists &L = t->lists[kind];
+    if (!L.built) return set_error(SPHK_E_STATE, "dtree list kind %d not built", kind);
+    if (kind == 2 && (!s_off || (!s_idx && L.n_s > 0)))
+        return set_error(SPHK_E_ARG, "dtree list copy: null SPJ array");
+    const size_t no = (size_t)t->n_walk + 1;
+    SPHK_TRY(hipMemcpyAsync(j_off, L.j_off.p, no * sizeof(int32_t), hipMemcpyDeviceToDevice, t->st));
+    if (kind == 2) SPHK_TRY(hipMemcpyAsync(s_off, L.s_off.p, no * sizeof(int32_t), hipMemcpyDeviceToDevice, t->st));
+    if (L.in_rooms) return pack_list(t, kind, j_idx, kind == 2 ? s_idx : nullptr);
+    if (L.n_j > 0)
+        SPHK_TRY(hipMemcpyAsync(j_idx, L.j_idx.p, (size_t)L.n_j * sizeof(int32_t), hipMemcpyDeviceToDevice, t->st));
+    if (kind == 2 && L.n_s > 0)
+        SPHK_TRY(hipMemcpyAsync(s_idx, L.s_idx.p, (size_t)L.n_s * sizeof(int32_t), hipMemcpyDeviceToDevice, t->st));
     return SPHK_OK;
 }
 

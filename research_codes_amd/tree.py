@@ -217,15 +217,19 @@ class DeviceTree:
             pass
 
     def _get(self, kind: int):
+        """The built list in new tensors (packed straight from the build's
+        rooms when it was built in one walk)."""
         import torch
-        jo, ji, so, si = _P(), _P(), _P(), _P()
         nj, ns = C.c_int64(), C.c_int64()
-        check(lib().sphk_dtree_list(self._h, kind, C.byref(jo), C.byref(ji), C.byref(so), C.byref(si),
-                                    C.byref(nj), C.byref(ns)), "sphk_dtree_list")
-        out = [_copy_out(jo.value, self.n_walk + 1, torch.int32), _copy_out(ji.value, nj.value, torch.int32)]
+        check(lib().sphk_dtree_list_size(self._h, kind, C.byref(nj), C.byref(ns)), "sphk_dtree_list_size")
+        t32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device="cuda")   # noqa: E731
+        jo, ji = t32(self.n_walk + 1), t32(nj.value)
+        so, si = (t32(self.n_walk + 1), t32(ns.value)) if kind == 2 else (None, None)
+        check(lib().sphk_dtree_list_copy(self._h, kind, _dptr(jo), _dptr(ji), _dptr(so), _dptr(si)),
+              "sphk_dtree_list_copy")
+        out = [jo, ji[:nj.value]]
         if kind == 2:
-            out += [_copy_out(so.value, self.n_walk + 1, torch.int32),
-                    _copy_out(si.value, ns.value, torch.int32)]
+            out += [so, si[:ns.value]]
         return tuple(out)
 
     def gather(self, rs):

@@ -217,3 +217,41 @@ def test_long_lists_beyond_the_walk_room():
     finally:
         ht.close()
         dt.close()
+
+
+def test_handle_arrays_of_lists_built_in_one_walk():
+    """A list built in one walk stays in its rooms until read: the handle's
+    own arrays (sphk_dtree_list) pack it, also after another list has been
+    built on the same handle (the rooms are reused), and equal the copy."""
+    import ctypes as C
+    p = sort_particles(_cases()["pair"])[0]
+    dt = DeviceTree(_d(p["x"]), _d(p["y"]), _d(p["z"]), _d(p["m"]), None, 8, 32)
+    try:
+        g = dt.gather(_d(1.1 * p["ksr"]))
+        lg = dt.long(0.5)
+
+        def handle_arrays(kind):
+            jo, ji, so, si = (C.c_void_p() for _ in range(4))
+            nj, ns = C.c_int64(), C.c_int64()
+            assert L.lib().sphk_dtree_list(dt._h, kind, C.byref(jo), C.byref(ji), C.byref(so), C.byref(si),
+                                           C.byref(nj), C.byref(ns)) == 0
+            out = [(jo.value, dt.n_walk + 1), (ji.value, nj.value)]
+            if kind == 2:
+                out += [(so.value, dt.n_walk + 1), (si.value, ns.value)]
+            res = []
+            for ptr, n in out:
+                t = torch.empty(n, dtype=torch.int32, device="cuda")
+                assert L.lib().sphk_copy_dev(t.data_ptr(), ptr, 4 * n, None) == 0
+                res.append(t)
+            torch.cuda.synchronize()
+            return res
+
+        for a, b in zip(handle_arrays(0), g):
+            assert torch.equal(a, b)
+        for a, b in zip(handle_arrays(2), lg):
+            assert torch.equal(a, b)
+        # read again after the handle packed them: the copy path
+        for a, b in zip(dt._get(2), lg):
+            assert torch.equal(a, b)
+    finally:
+        dt.close()
