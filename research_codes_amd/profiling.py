@@ -110,10 +110,11 @@ def counters(csv_path: str, kernel: str, last: int = 5) -> dict:
 
 
 _KDK_STAGES = (
-    ("long list (EPJ + SPJ): count + fill", ("list_kernel<2",)),
-    ("symmetry list: candidates count + fill, exact members", ("list_kernel<1", "member_kernel<1")),
-    ("gather list: candidates count + fill, exact members", ("list_kernel<0", "member_kernel<0")),
-    ("list bookkeeping (walk boxes, offsets, member copies)", ("walk_box", "offsets_kernel", "member_copy")),
+    ("long list (EPJ + SPJ): one walk into rooms (count + fill on overflow)", ("list_kernel<2",)),
+    ("symmetry list: candidates in one walk, exact members", ("list_kernel<1", "member_kernel<1")),
+    ("gather list: candidates in one walk, exact members", ("list_kernel<0", "member_kernel<0")),
+    ("list bookkeeping (walk boxes, offsets, packing the lists)", ("walk_box", "offsets_kernel", "member_copy",
+                                                                   "room_copy")),
     ("scans and sorts (rocPRIM)", ("rocprim",)),
     ("tree rebuild (keys, levels, moments, node records, groups)",
      ("split_kernel", "moments_kernel", "obox_kernel", "next_kernel", "pack_", "bbox_", "keys_kernel",
