@@ -90,6 +90,12 @@ typedef struct {
      * queues of the context: at most 64 persistent launches of one context
      * may be in flight at once. */
     int persistent;
+    /* Host dispatch (sphk_*_dispatch): 1 declares that the caller's source
+     * arrays are pinned host memory (hipHostMalloc) left unchanged until the
+     * matching sphk_*_retrieve returns; they are then copied to the device
+     * directly instead of through the tag's pinned staging block (one host
+     * copy of every field less).  0 (default): staged. */
+    int pinned_src;
 } sphk_params;
 
 /* ---- inputs / outputs (all FP64 unless stated) ------------------------ */

@@ -29,7 +29,8 @@ class SphkError(RuntimeError):
 
 class Params(C.Structure):
     _fields_ = [("eta", C.c_double), ("ksrh", C.c_double), ("hmax", C.c_double),
-                ("uniform_softening", C.c_int), ("list_cap", C.c_int), ("persistent", C.c_int)]
+                ("uniform_softening", C.c_int), ("list_cap", C.c_int), ("persistent", C.c_int),
+                ("pinned_src", C.c_int)]
 
 
 DENS_SRC = ("id", "m", "x", "y", "z", "vx", "vy", "vz", "ksr", "rs")

@@ -3,17 +3,22 @@
 // ABI of libsphk.so.
 //
 // Dispatch copies each walk's AoS EPI/EPJ arrays (owned by FDPS, valid only
-// during the call) into per-tag structure-of-arrays host buffers, walks in
-// parallel (OpenMP), and calls
-// sphk_*_dispatch, which stages them into pinned memory, copies them to the
-// device in one transfer and launches asynchronously.  Retrieve waits for
+// during the call) into per-tag structure-of-arrays host buffers in pinned
+// memory, walks in parallel (OpenMP), and calls sphk_*_dispatch, which
+// copies them to the device straight from those buffers (the context is set
+// to pinned_src; FDPS retrieves a tag before it dispatches on it again, so
+// the buffers stay unchanged while a copy is in flight) and launches
+// asynchronously.  Retrieve waits for
 // the result and scatters it back into FDPS's per-walk Force arrays in walk
 // order (as hdr_densityGPU.cu:594-621 does).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "sphk.h"
 #include "sphk_fdps.hpp"
@@ -45,9 +50,40 @@ void need_ctx()
     }
 }
 
+// Host memory for the packed columns: pinned, so the dispatch copies them to
+// the device directly.
+template <typename T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <typename U>
+    PinnedAlloc(const PinnedAlloc<U> &)
+    {
+    }
+    T *allocate(size_t n)
+    {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t) { (void)hipHostFree(p); }
+};
+template <typename T, typename U>
+bool operator==(const PinnedAlloc<T> &, const PinnedAlloc<U> &)
+{
+    return true;
+}
+template <typename T, typename U>
+bool operator!=(const PinnedAlloc<T> &, const PinnedAlloc<U> &)
+{
+    return false;
+}
+template <typename T>
+using PinnedVec = std::vector<T, PinnedAlloc<T>>;
+
 struct Cols {
-    std::vector<std::vector<double>> f;
-    std::vector<int32_t> id;
+    std::vector<PinnedVec<double>> f;
+    PinnedVec<int32_t> id;
     void resize(size_t nf, size_t n)
     {
         f.resize(nf);
@@ -65,7 +101,9 @@ struct TagBuf {
     std::vector<int64_t> onp;
     std::vector<uint8_t> oitr;
 };
-TagBuf g_tag[SPHK_TAG_MAX];
+// Never destroyed: its pinned buffers are freed by reset_gpu, or left to
+// the process's end (not freed after the HIP runtime has shut down).
+TagBuf *g_tag = new TagBuf[SPHK_TAG_MAX];
 
 TagBuf &tagbuf(int tag)
 {
@@ -95,6 +133,13 @@ void setup_gpu(int my_proc)
     if (g_ctx) return;
     int rc = sphk_init(my_proc, &g_ctx);
     if (rc) die("setup_gpu", rc);
+    // The tag buffers are pinned: copied to the device directly, unless
+    // SPHK_STAGED=1 asks for the context's staging blocks.
+    const char *st = std::getenv("SPHK_STAGED");
+    sphk_params prm;
+    sphk_get_params(g_ctx, &prm);
+    prm.pinned_src = (st && std::strcmp(st, "1") == 0) ? 0 : 1;
+    if ((rc = sphk_set_params(g_ctx, &prm))) die("setup_gpu", rc);
     const char *pr = std::getenv("SPHK_PRECISION");
     g_precision = (pr && std::strcmp(pr, "fp64") == 0) ? SPHK_FP64 : SPHK_MIXED;
     if (my_proc == 0 && std::getenv("SPHK_VERBOSE"))
@@ -108,6 +153,7 @@ void reset_gpu(int)
     std::lock_guard<std::mutex> g(g_mu);
     if (g_ctx) sphk_finalize(g_ctx);
     g_ctx = nullptr;
+    for (int t = 0; t < SPHK_TAG_MAX; t++) g_tag[t] = TagBuf();
 }
 
 int DispatchKernel_Dens(const int tag, const int n_walk, const DensityEPI **epi, const int *nip,

@@ -141,7 +141,7 @@ constexpr int kQueueSlots = 64;
 struct sphk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    sphk_params params{1.6, 1.936492, 1e30, 0, 0, 0};
+    sphk_params params{1.6, 1.936492, 1e30, 0, 0, 0, 0};
     int *queue = nullptr;          // kQueueSlots * kQueueInts ints on the device
     unsigned queue_next = 0;
     sphk::TagSlot tags[SPHK_TAG_MAX];
@@ -228,26 +228,37 @@ int prefix(int n_walk, const int32_t *cnt, std::vector<int32_t> &off, size_t lim
     return SPHK_OK;
 }
 
-// Stage: offsets arrays followed by each non-null field of each src struct.
+// Stage: offsets arrays followed by each non-null field of each src struct
+// (caller: the field is one of the caller's arrays).
 struct StageItem {
     const void *src;
     size_t bytes;
     size_t off;
+    bool caller = false;
 };
 
+// One device block per tag holds the items.  They go through the tag's
+// pinned staging block in one transfer, or, for the caller's arrays when the
+// caller declared them pinned (params.pinned_src), straight from them.
 int stage_and_copy(sphk_ctx *ctx, TagSlot &slot, std::vector<StageItem> &items)
 {
-    size_t total = 0;
+    const bool direct = ctx->params.pinned_src != 0;
+    size_t total = 0, staged = 0;
     for (auto &it : items) {
         it.off = total;
         total += align256(it.bytes);
+        if (!(direct && it.caller)) staged = total;   // staged items lead the block
     }
     SPHK_HIP(hipStreamSynchronize(ctx->stream));  // staging block may still be in flight
     SPHK_HIP(slot.hin.ensure(total));
     SPHK_HIP(slot.din.ensure(total));
     for (auto &it : items)
-        if (it.bytes) std::memcpy(slot.hin.p + it.off, it.src, it.bytes);
-    SPHK_HIP(hipMemcpyAsync(slot.din.p, slot.hin.p, total, hipMemcpyHostToDevice, ctx->stream));
+        if (it.bytes && !(direct && it.caller)) std::memcpy(slot.hin.p + it.off, it.src, it.bytes);
+    if (staged) SPHK_HIP(hipMemcpyAsync(slot.din.p, slot.hin.p, staged, hipMemcpyHostToDevice, ctx->stream));
+    if (direct)
+        for (auto &it : items)
+            if (it.caller && it.bytes)
+                SPHK_HIP(hipMemcpyAsync(slot.din.p + it.off, it.src, it.bytes, hipMemcpyHostToDevice, ctx->stream));
     return SPHK_OK;
 }
 
@@ -259,7 +270,7 @@ int add_fields(std::vector<StageItem> &items, const S &s, const size_t *sz, int 
     for (int k = 0; k < nf; k++) {
         if (f[k]) {
             slot_of.push_back((int)items.size());
-            items.push_back({f[k], sz[k] * rows, 0});
+            items.push_back({f[k], sz[k] * rows, 0, true});
         } else {
             slot_of.push_back(-1);
         }
@@ -398,6 +409,8 @@ int sphk_set_params(sphk_ctx *ctx, const sphk_params *p)
         return set_error(SPHK_E_ARG, "list_cap must be 0, 192 or 256 (got %d)", p->list_cap);
     if (p->persistent != 0 && p->persistent != 1)
         return set_error(SPHK_E_ARG, "persistent must be 0 or 1 (got %d)", p->persistent);
+    if (p->pinned_src != 0 && p->pinned_src != 1)
+        return set_error(SPHK_E_ARG, "pinned_src must be 0 or 1 (got %d)", p->pinned_src);
     ctx->params = *p;
     return SPHK_OK;
 }

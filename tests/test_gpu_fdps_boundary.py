@@ -4,7 +4,9 @@ signatures of sph_yamauchi/run/use_gpu.hpp:16-64) driven by a C++ program that
 batches walks the way FDPS does (<= 200 walks per Dispatch, tag 0, per-walk
 AoS copies) must reproduce the oracle on the same walks and lists:
 np/itr bit-exact; FP64 (SPHK_PRECISION=fp64) reals within 1e-5; mixed within
-1e-4 (zero-crossing sums against max(|ref|, 0.1 rms)).
+1e-4 (zero-crossing sums against max(|ref|, 0.1 rms)).  Both host routes: the
+adapter's pinned buffers copied directly (default) and through the context's
+staging blocks (SPHK_STAGED=1).
 """
 import os
 import subprocess
@@ -38,8 +40,9 @@ def case(request):
     return polytrope_case(N, seed=77 + N, n_group=g)
 
 
+@pytest.mark.parametrize("route", ["direct", "staged"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_fdps_dispatch_retrieve_matches_oracle(case, prec, tmp_path):
+def test_fdps_dispatch_retrieve_matches_oracle(case, prec, route, tmp_path):
     assert DRIVER.exists(), "tests/fdps_driver is built by __graft_entry__.build()"
     p, h, g = case["p"], case["hydro_src"], case["grav_src"]
     jl = case["grav_lists"]
@@ -61,8 +64,8 @@ def test_fdps_dispatch_retrieve_matches_oracle(case, prec, tmp_path):
               "grav_i_off", "grav_j_off", "grav_j_idx", "grav_s_off", "grav_s_idx"):
         arrays[k] = np.asarray(arrays[k], np.int32)
     _write(tmp_path, arrays)
-    env = dict(os.environ, SPHK_PRECISION=prec)
-    print(f"[fdps] driver N={len(p['m'])} prec={prec}", flush=True)
+    env = dict(os.environ, SPHK_PRECISION=prec, SPHK_STAGED="1" if route == "staged" else "0")
+    print(f"[fdps] driver N={len(p['m'])} prec={prec} route={route}", flush=True)
     r = subprocess.run([str(DRIVER), str(tmp_path)], env=env, capture_output=True, text=True,
                        timeout=240)
     assert r.returncode == 0, r.stdout + r.stderr
