@@ -642,18 +642,17 @@ __device__ __forceinline__ int walk_scan_incl(int v, int c)
 // as the host's leaf loop decides it: MODE 0 |x_j - x_i| < rs_i for some i of
 // the walk; MODE 1 r_ij < max(H_i, H_j) for some i, after the box pre-tests
 // (j inside the walk's search box, or j's sphere reaching the group's box).
-// The walk's i-particles (x, y, z, r) are read from si, the first kWalkI of
-// them staged in LDS.
+// j's position is p (and its H hj in MODE 1).  The walk's i-particles
+// (x, y, z, r) are read from si, the first nl of them staged in LDS.
 constexpr int kWalkI = 256;
 
 template <int MODE>
-__device__ __forceinline__ bool member(const TreeView &t, const double *__restrict__ r, int j, int ib, int ni,
-                                       const double4 *si, const double (&olo)[3], const double (&ohi)[3],
-                                       const double (&glo)[3], const double (&ghi)[3])
+__device__ __forceinline__ bool member(const TreeView &t, const double *__restrict__ r, const double (&p)[3],
+                                       double hj, int ib, int ni, int nl, const double4 *si,
+                                       const double (&olo)[3], const double (&ohi)[3], const double (&glo)[3],
+                                       const double (&ghi)[3])
 {
 #pragma clang fp contract(off)
-    const double p[3] = {t.x[j], t.y[j], t.z[j]};
-    const double hj = MODE == 1 ? r[j] : 0.;
     bool in = true, jov = MODE == 1;
     for (int d = 0; d < 3; d++) {
         in = in && !(p[d] < olo[d] || p[d] > ohi[d]);
@@ -665,7 +664,6 @@ __device__ __forceinline__ bool member(const TreeView &t, const double *__restri
         const double rr = MODE == 1 ? (P.w > hj ? P.w : hj) : P.w;
         return dx * dx + dy * dy + dz * dz < rr * rr;
     };
-    const int nl = ni < kWalkI ? ni : kWalkI;
     int q = 0;
     for (; q + 2 <= nl; q += 2) {   // LDS, two at a time
         const double4 P0 = si[q], P1 = si[q + 1];
@@ -815,23 +813,28 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
 }
 
 // Exact membership of each walk's candidates (the particles of the leaves its
-// walk reached, in the host's order), one wave per walk: the walk's
-// i-particles sit in LDS, lane l tests candidate c0 + l, and the members are
-// compacted in place (in order) at the front of the walk's stretch of scr;
-// their number goes to cnt.
+// walk reached, in the host's order), one wave per walk: the first n_si
+// (<= kWalkI) i-particles of the walk sit in LDS (dynamic, n_si double4), any
+// beyond are read from memory; lane l tests candidate c0 + l, and the members
+// are compacted in place (in order) at the front of the walk's stretch of
+// scr; their number goes to cnt.  Candidates are taken kMemberU * 64 at a
+// time: their indices and positions are all loaded before any is tested.
+constexpr int kMemberU = 4;
+
 template <int MODE>
 __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const double *__restrict__ wb,
                                                     const double *__restrict__ r, const int64_t *__restrict__ coff,
                                                     const int64_t *__restrict__ ccnt, int32_t *__restrict__ scr,
-                                                    int64_t *__restrict__ cnt)
+                                                    int64_t *__restrict__ cnt, int n_si)
 {
-    __shared__ double4 si[kWalkI];
+    extern __shared__ double4 si[];
     const int g = blockIdx.x;
     if (g >= ng) return;
     const int lane = threadIdx.x;
     const size_t C = t.mo.cap;
     const int ib = t.i_off[g], ni = t.i_off[g + 1] - ib;
-    for (int q = lane; q < ni && q < kWalkI; q += 64)
+    const int nl = ni < n_si ? ni : n_si;
+    for (int q = lane; q < nl; q += 64)
         si[q] = make_double4(t.x[ib + q], t.y[ib + q], t.z[ib + q], r[ib + q]);
     const int gn = t.group[g];
     double olo[3], ohi[3], glo[3], ghi[3];
@@ -845,14 +848,30 @@ __global__ __launch_bounds__(64) void member_kernel(int ng, TreeView t, const do
     int32_t *s = scr + coff[g];
     const int64_t nc = ccnt[g];
     int64_t count = 0;
-    for (int64_t c0 = 0; c0 < nc; c0 += 64) {
-        const int64_t e = c0 + lane;
-        const int j = e < nc ? s[e] : 0;
-        const bool acc = e < nc && member<MODE>(t, r, j, ib, ni, si, olo, ohi, glo, ghi);
-        const uint64_t bal = __ballot(acc);
-        // writes land at or before the chunk's own (already read) slots
-        if (acc) s[count + __popcll(bal & ((1ull << lane) - 1ull))] = j;
-        count += __popcll(bal);
+    for (int64_t c0 = 0; c0 < nc; c0 += 64 * kMemberU) {
+        int j[kMemberU];
+        double p[kMemberU][3], hj[kMemberU];
+#pragma unroll
+        for (int u = 0; u < kMemberU; u++) {
+            const int64_t e = c0 + 64 * u + lane;
+            j[u] = e < nc ? s[e] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kMemberU; u++) {
+            const int k = j[u] < 0 ? 0 : j[u];
+            p[u][0] = t.x[k];
+            p[u][1] = t.y[k];
+            p[u][2] = t.z[k];
+            hj[u] = MODE == 1 ? r[k] : 0.;
+        }
+        // every slot of this stretch is read: writes land at or before them
+#pragma unroll
+        for (int u = 0; u < kMemberU; u++) {
+            const bool acc = j[u] >= 0 && member<MODE>(t, r, p[u], hj[u], ib, ni, nl, si, olo, ohi, glo, ghi);
+            const uint64_t bal = __ballot(acc);
+            if (acc) s[count + __popcll(bal & ((1ull << lane) - 1ull))] = j[u];
+            count += __popcll(bal);
+        }
     }
     if (lane == 0) cnt[g] = count;
 }
@@ -1059,6 +1078,9 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
     L.built = false;
     L.in_rooms = false;
     const int ng = t->n_walk;
+    // i-particles staged in LDS by the member test: a walk holds <= n_group
+    // particles unless it is a leaf (<= n_leaf; more only at the deepest level)
+    const int n_si = std::min(kWalkI, std::max(t->n_group, t->n_leaf));
     SPHK_TRY(t->i64a.reserve(ng, false, t->st));   // EPJ counts per walk
     SPHK_TRY(t->i64b.reserve(ng, false, t->st));   // SPJ counts
     SPHK_TRY(t->i64c.reserve(ng, false, t->st));   // EPJ offsets
@@ -1085,8 +1107,8 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
             SPHK_TRY(t->mscr.reserve((size_t)ng * (size_t)room, false, t->st));
             hipLaunchKernelGGL((walk_list_kernel<MODE, 2, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r, inv_t2,
                                t->i64a.p, t->i64c.p, nullptr, nullptr, t->mscr.p, nullptr, t->err.p, room);
-            hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, wb, r, t->i64c.p,
-                               t->i64a.p, t->mscr.p, t->i64b.p);
+            hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), n_si * sizeof(double4), t->st, ng, v, wb,
+                               r, t->i64c.p, t->i64a.p, t->mscr.p, t->i64b.p, n_si);
             SPHK_TRY(hipGetLastError());
             if (int rc = walk_offsets(t, ng, t->i64b.p, t->i64d.p, L.j_off.p, nullptr, nullptr, nullptr)) return rc;
             int64_t tot1[2] = {0, 0};
@@ -1186,8 +1208,8 @@ int build_list(sphk_dtree *t, const double *r, double inv_t2)
         else
             hipLaunchKernelGGL((walk_list_kernel<MODE, 1, kListStackSmall>), lg, lb, 0, t->st, ng, v, wb, r,
                                inv_t2, nullptr, nullptr, t->i64c.p, nullptr, t->mscr.p, nullptr, t->err.p);
-        hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), 0, t->st, ng, v, wb, r, t->i64c.p, t->i64a.p,
-                           t->mscr.p, t->i64b.p);
+        hipLaunchKernelGGL((member_kernel<MODE>), dim3(ng), dim3(64), n_si * sizeof(double4), t->st, ng, v, wb, r,
+                           t->i64c.p, t->i64a.p, t->mscr.p, t->i64b.p, n_si);
         SPHK_TRY(hipGetLastError());
         if (int rc = walk_offsets(t, ng, t->i64b.p, t->i64d.p, L.j_off.p, nullptr, nullptr, nullptr)) return rc;
         if (int rc = read_totals(t, tot, overflow)) return rc;

@@ -113,6 +113,34 @@ def test_deep_tree_lists_overflowing_the_small_stack():
         dt.close()
 
 
+def test_walks_beyond_the_staged_i_particles():
+    """40 coincident particles share one leaf at the deepest level, a walk of
+    40 > max(n_group, n_leaf) = 8 particles: the member test stages 8 of a
+    walk's i-particles and reads the rest from memory; the lists must still
+    equal the host's."""
+    rng = np.random.default_rng(3)
+    n = 600
+    x, y, z = rng.uniform(0.0, 1.0, (3, n))
+    x[:40], y[:40], z[:40] = 0.37, 0.41, 0.53
+    perm = morton_order(x, y, z)
+    x, y, z = x[perm], y[perm], z[perm]
+    m, eps2, h = np.full(n, 1.0 / n), np.full(n, 1e-6), rng.uniform(0.1, 0.2, n)
+    ht = HostTree(x, y, z, m, eps2, 1, 8)
+    dt = DeviceTree(_d(x), _d(y), _d(z), _d(m), _d(eps2), 1, 8)
+    try:
+        assert np.diff(ht.i_off).max() >= 40
+        assert dt.n_walk == ht.n_walk and np.array_equal(dt.i_off.cpu().numpy(), ht.i_off)
+        for name, hl, dl in (("gather", ht.gather(h), dt.gather(_d(h))),
+                             ("symmetry", ht.symmetry(h), dt.symmetry(_d(h))),
+                             ("long0.5", ht.long(0.5), dt.long(0.5))):
+            assert len(hl) == len(dl), name
+            for a, b in zip(hl, dl):
+                assert np.array_equal(a, b.cpu().numpy()), name
+    finally:
+        ht.close()
+        dt.close()
+
+
 def test_rebuild_in_place_matches_fresh_build():
     """One handle rebuilt for a sequence of particle sets (larger, smaller,
     another leaf and group size) gives exactly what a new handle gives."""
