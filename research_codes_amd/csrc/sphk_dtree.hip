@@ -447,37 +447,19 @@ __global__ __launch_bounds__(kBlock) void group_write_kernel(int n, const int32_
 // one 64-byte record per node (one memory segment per visit instead of one
 // per field array).  Values are copies of the node arrays, so the walks'
 // comparisons and their results are unchanged.
-// `next` is the node the depth-first walk visits after a's subtree (its
-// next sibling, else its parent's next, -1 after the last): the walks need
-// no stack.
 struct alignas(64) BoxRec {     // gather and symmetry walks
     double lo[3], hi[3];        // tight box
-    int32_t child, next, begin, end;
+    int32_t child, nchild, begin, end;
 };
 struct alignas(64) MacRec {     // long walk
     double mx, my, mz, mm, len;
-    int32_t child, next, begin, end;
+    int32_t child, nchild, begin, end;
 };
 struct alignas(64) OboxRec {    // symmetry walk: box of the nodes' particle spheres
     double lo[3], hi[3];
 };
 
-// Skip pointers of one level, parents' already set (levels top-down).
-__global__ __launch_bounds__(kBlock) void next_kernel(int a0, int na, NodeArrays nd, int32_t *__restrict__ nxt)
-{
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= na) return;
-    const int a = a0 + k;
-    const int p = nd.parent[a];
-    if (p < 0) {
-        nxt[a] = -1;
-        return;
-    }
-    nxt[a] = a < nd.child[p] + nd.nchild[p] - 1 ? a + 1 : nxt[p];
-}
-
 __global__ __launch_bounds__(kBlock) void pack_nodes_kernel(int nn, NodeArrays nd, Moments mo,
-                                                            const int32_t *__restrict__ nxt,
                                                             BoxRec *__restrict__ br, MacRec *__restrict__ mr)
 {
     const int a = blockIdx.x * kBlock + threadIdx.x;
@@ -489,11 +471,11 @@ __global__ __launch_bounds__(kBlock) void pack_nodes_kernel(int nn, NodeArrays n
         b.hi[d] = mo.hi[d * C + a];
     }
     b.child = nd.child[a];
-    b.next = nxt[a];
+    b.nchild = nd.nchild[a];
     b.begin = nd.begin[a];
     b.end = nd.end[a];
     br[a] = b;
-    mr[a] = MacRec{mo.mx[a], mo.my[a], mo.mz[a], mo.mm[a], nd.len[a], b.child, b.next, b.begin, b.end};
+    mr[a] = MacRec{mo.mx[a], mo.my[a], mo.mz[a], mo.mm[a], nd.len[a], b.child, b.nchild, b.begin, b.end};
 }
 
 __global__ __launch_bounds__(kBlock) void pack_obox_kernel(int nn, const double *__restrict__ olo,
@@ -558,7 +540,7 @@ enum Visit { kSkip, kSpj, kLeaf, kOpen };
 
 // Child range and particle range of the visited node, from its record.
 struct NodeLinks {
-    int32_t child, next, begin, end;
+    int32_t child, nchild, begin, end;
 };
 
 template <int MODE>
@@ -569,20 +551,21 @@ __device__ __forceinline__ Visit visit(const TreeView &t, int a, const double (&
 #pragma clang fp contract(off)
     if (MODE != 2) {
         const BoxRec b = t.br[a];
-        ln = NodeLinks{b.child, b.next, b.begin, b.end};
+        OboxRec o{};
+        if (MODE == 1) o = t.orr[a];   // loaded with b: one memory round trip per visit
+        ln = NodeLinks{b.child, b.nchild, b.begin, b.end};
         const bool leaf = b.child < 0;
         if (MODE == 0) {
             if (!box_overlaps(olo, ohi, b.lo, b.hi)) return kSkip;
             return leaf ? kLeaf : kOpen;
         }
         if (!box_overlaps(olo, ohi, b.lo, b.hi)) {
-            const OboxRec o = t.orr[a];
             if (!box_overlaps(glo, ghi, o.lo, o.hi)) return kSkip;
         }
         return leaf ? kLeaf : kOpen;
     }
     const MacRec m = t.mr[a];
-    ln = NodeLinks{m.child, m.next, m.begin, m.end};
+    ln = NodeLinks{m.child, m.nchild, m.begin, m.end};
     if (m.mm == 0.) return kSkip;
     const double c[3] = {m.mx, m.my, m.mz};
     double s = 0.;
@@ -767,7 +750,7 @@ __global__ __launch_bounds__(64) void walk_list_kernel(int ng, TreeView t, const
                 w1 = kTyLeaf | (uint32_t)(ln.end - ln.begin);
             } else if (v == kOpen) {
                 w0 = (uint32_t)ln.child;
-                w1 = kTyOpen | (uint32_t)t.nd.nchild[a];
+                w1 = kTyOpen | (uint32_t)ln.nchild;
             }
         }
         const bool kept = ev && w1 != 0;
@@ -930,7 +913,7 @@ struct sphk_dtree {
     std::vector<int> level_start;   // node index of the first node of each level
     int n_node = 0;
     DArr<double> box_lo, box_hi, mm, mx, my, mz, me, sx, sy, sz, se, olo, ohi;
-    DArr<int32_t> head, group, i_off, mscr, nextp;
+    DArr<int32_t> head, group, i_off, mscr;
     int n_walk = 0;
     DArr<double> wbox;
     DArr<BoxRec> brec;
@@ -959,7 +942,7 @@ struct sphk_dtree {
                         &sz, &se, &olo, &ohi})
             a->release();
         key.release();
-        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &mscr, &nextp})
+        for (auto *a : {&begin, &end, &child, &nchild, &level, &parent, &head, &group, &i_off, &mscr})
             a->release();
         wbox.release();
         brec.release();
@@ -1364,15 +1347,10 @@ int build_tree(sphk_dtree *t, int n, const double *x, const double *y, const dou
                            t->x.p, t->y.p, t->z.p, m, eps2);
     }
     if (hipGetLastError() != hipSuccess) return fail(set_error(SPHK_E_DEVICE, "dtree: moments launch failed"));
-    if (t->brec.reserve(nn, false, st) != hipSuccess || t->mrec.reserve(nn, false, st) != hipSuccess ||
-        t->nextp.reserve(nn, false, st) != hipSuccess)
+    if (t->brec.reserve(nn, false, st) != hipSuccess || t->mrec.reserve(nn, false, st) != hipSuccess)
         return fail(set_error(SPHK_E_DEVICE, "dtree: allocation failed"));
-    for (int L = 0; L + 1 < (int)t->level_start.size(); L++) {
-        const int a0 = t->level_start[L], na = t->level_start[L + 1] - a0;
-        hipLaunchKernelGGL(next_kernel, dim3(grid(na)), dim3(kBlock), 0, st, a0, na, t->nodes(), t->nextp.p);
-    }
     hipLaunchKernelGGL(pack_nodes_kernel, dim3(grid((int)nn)), dim3(kBlock), 0, st, (int)nn, t->nodes(),
-                       t->moments(), t->nextp.p, t->brec.p, t->mrec.p);
+                       t->moments(), t->brec.p, t->mrec.p);
 
     // groups
     if (t->head.reserve(n, false, st) != hipSuccess || t->i64a.reserve(n, false, st) != hipSuccess ||
