@@ -96,6 +96,22 @@ int sphk_dtree_nodes(const sphk_dtree *t, const int32_t **begin, const int32_t *
  * results in device buffers of their own). */
 int sphk_copy_dev(void *dst, const void *src, size_t bytes, void *stream);
 
+/* Reorder particle data in place after a Morton sort: for each of `count`
+ * (<= 64) columns, col_c[k] = old col_c[perm[k]], k < n, where perm is a
+ * permutation of 0..n-1 (as sphk_morton_order_dev writes) and column c holds
+ * n elements of elem[c] = 1, 2, 4 or 8 bytes.  `work` is device scratch of at
+ * least sphk_permute_work_bytes(n, count, elem) bytes, 256-byte aligned.
+ * Asynchronous on `stream` (two launches for all columns). */
+size_t sphk_permute_work_bytes(int n, int count, const int *elem);
+int sphk_permute_dev(int n, const int32_t *perm, int count, void *const *cols, const int *elem, void *work,
+                     size_t work_bytes, void *stream);
+
+/* *flag (device int32) = 1 if any of the `count` (<= 8) pairs of n-element
+ * FP64 device arrays a[c], b[c] differ in any bit, else 0 (the list rebuilds'
+ * test for unchanged inputs).  Asynchronous on `stream`. */
+int sphk_equal_dev(int n, int count, const double *const *a, const double *const *b, int32_t *flag,
+                   void *stream);
+
 #ifdef __cplusplus
 }
 #endif

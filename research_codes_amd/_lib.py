@@ -144,6 +144,10 @@ SIGNATURES.update({
     "sphk_dtree_moments": (C.c_int, [_P] + [C.POINTER(_P)] * 5),
     "sphk_dtree_nodes": (C.c_int, [_P] + [C.POINTER(_P)] * 6),
     "sphk_copy_dev": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "sphk_permute_work_bytes": (C.c_size_t, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "sphk_permute_dev": (C.c_int, [C.c_int, _P, C.c_int, C.POINTER(_P), C.POINTER(C.c_int), _P, C.c_size_t,
+                                   _P]),
+    "sphk_equal_dev": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P), C.POINTER(_P), _P, _P]),
     "sphk_unbound_mass_dev": (C.c_int, [_P, C.c_int, State, _P, C.c_int, C.c_double, _DP, _P]),
     # sphk_adjoint.h
     "sphk_gravity_vjp_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, GravSrc, _P, _P, GravSrc, GravSrc, C.c_double]

@@ -1600,7 +1600,134 @@ extern "C" int sphk_dtree_spread_dev(const sphk_dtree *t, const double *m, const
     return SPHK_OK;
 }
 
+namespace {
+
+// ---- reorder and input comparison (host glue of the rebuilds) ----------------
+constexpr int kPermCols = 64;
+struct PermCols {
+    char *col[kPermCols];
+    char *tmp[kPermCols];
+    int elem[kPermCols];
+};
+
+template <typename T>
+__device__ __forceinline__ void gather_col(int n, const int32_t *__restrict__ perm, const char *src, char *dst)
+{
+    const T *s = reinterpret_cast<const T *>(src);
+    T *d = reinterpret_cast<T *>(dst);
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) d[k] = s[perm ? perm[k] : k];
+}
+
+// tmp_c[k] = col_c[perm[k]], or with perm == nullptr col_c[k] = tmp_c[k];
+// grid row c handles column c
+__global__ __launch_bounds__(kBlock) void permute_kernel(int n, const int32_t *__restrict__ perm, PermCols pc)
+{
+    const int c = blockIdx.y;
+    const char *src = perm ? pc.col[c] : pc.tmp[c];
+    char *dst = perm ? pc.tmp[c] : pc.col[c];
+    if (!perm && ((uintptr_t)dst % 16) == 0) {
+        // copy back 16 bytes per thread (the scratch is 256-byte aligned)
+        const size_t bytes = (size_t)n * pc.elem[c];
+        const size_t n16 = bytes / 16;
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+        uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+        const size_t stride = (size_t)gridDim.x * kBlock;
+        for (size_t k = blockIdx.x * kBlock + threadIdx.x; k < n16; k += stride) d4[k] = s4[k];
+        for (size_t k = n16 * 16 + blockIdx.x * kBlock + threadIdx.x; k < bytes; k += stride) dst[k] = src[k];
+        return;
+    }
+    // gather, or a copy back by elements into a column aligned to less than 16 bytes
+    switch (pc.elem[c]) {
+    case 8: gather_col<uint64_t>(n, perm, src, dst); break;
+    case 4: gather_col<uint32_t>(n, perm, src, dst); break;
+    case 2: gather_col<uint16_t>(n, perm, src, dst); break;
+    default: gather_col<uint8_t>(n, perm, src, dst); break;
+    }
+}
+
+constexpr int kEqualArrays = 8;
+struct EqualArrays {
+    const uint64_t *a[kEqualArrays];
+    const uint64_t *b[kEqualArrays];
+};
+
+__global__ __launch_bounds__(kBlock) void equal_kernel(int n, EqualArrays ea, int32_t *__restrict__ flag)
+{
+    const int c = blockIdx.y;
+    bool diff = false;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        diff = diff || ea.a[c][k] != ea.b[c][k];
+    if (__any(diff) && threadIdx.x % 64 == 0) *flag = 1;   // every writer stores 1
+}
+
+__global__ void flag_zero_kernel(int32_t *flag) { *flag = 0; }
+
+size_t permute_work(int n, int count, const int *elem)
+{
+    size_t w = 0;
+    for (int c = 0; c < count; c++) w += (((size_t)n * (size_t)elem[c]) + 255) / 256 * 256;
+    return w;
+}
+
+}  // namespace
+
 extern "C" {
+
+size_t sphk_permute_work_bytes(int n, int count, const int *elem)
+{
+    if (n <= 0 || count <= 0 || count > kPermCols || !elem) return 0;
+    return permute_work(n, count, elem);
+}
+
+int sphk_permute_dev(int n, const int32_t *perm, int count, void *const *cols, const int *elem, void *work,
+                     size_t work_bytes, void *stream)
+{
+    if (n < 0 || count < 0 || count > kPermCols) return set_error(SPHK_E_ARG, "permute_dev: bad sizes");
+    if (n == 0 || count == 0) return SPHK_OK;
+    if (!perm || !cols || !elem || !work) return set_error(SPHK_E_ARG, "permute_dev: null pointer");
+    PermCols pc{};
+    size_t off = 0;
+    for (int c = 0; c < count; c++) {
+        if (!cols[c]) return set_error(SPHK_E_ARG, "permute_dev: null column %d", c);
+        if (elem[c] != 1 && elem[c] != 2 && elem[c] != 4 && elem[c] != 8)
+            return set_error(SPHK_E_ARG, "permute_dev: element size %d of column %d", elem[c], c);
+        if (((uintptr_t)cols[c]) % (uintptr_t)elem[c])
+            return set_error(SPHK_E_ARG, "permute_dev: column %d misaligned", c);
+        pc.col[c] = static_cast<char *>(cols[c]);
+        pc.tmp[c] = static_cast<char *>(work) + off;
+        pc.elem[c] = elem[c];
+        off += (((size_t)n * (size_t)elem[c]) + 255) / 256 * 256;
+    }
+    if (work_bytes < off) return set_error(SPHK_E_CAPACITY, "permute_dev: work of %zu bytes, %zu needed", work_bytes, off);
+    if (((uintptr_t)work) % 256) return set_error(SPHK_E_ARG, "permute_dev: work not 256-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(std::min(1024, (n + kBlock - 1) / kBlock), count);
+    hipLaunchKernelGGL(permute_kernel, grid, dim3(kBlock), 0, st, n, perm, pc);
+    hipLaunchKernelGGL(permute_kernel, grid, dim3(kBlock), 0, st, n, nullptr, pc);
+    SPHK_TRY(hipGetLastError());
+    return SPHK_OK;
+}
+
+int sphk_equal_dev(int n, int count, const double *const *a, const double *const *b, int32_t *flag, void *stream)
+{
+    if (n < 0 || count < 0 || count > kEqualArrays) return set_error(SPHK_E_ARG, "equal_dev: bad sizes");
+    if (!flag) return set_error(SPHK_E_ARG, "equal_dev: null flag");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(flag_zero_kernel, dim3(1), dim3(1), 0, st, flag);
+    if (n > 0 && count > 0) {
+        if (!a || !b) return set_error(SPHK_E_ARG, "equal_dev: null array list");
+        EqualArrays ea{};
+        for (int c = 0; c < count; c++) {
+            if (!a[c] || !b[c]) return set_error(SPHK_E_ARG, "equal_dev: null array %d", c);
+            ea.a[c] = reinterpret_cast<const uint64_t *>(a[c]);
+            ea.b[c] = reinterpret_cast<const uint64_t *>(b[c]);
+        }
+        const dim3 grid(std::min(512, (n + kBlock - 1) / kBlock), count);
+        hipLaunchKernelGGL(equal_kernel, grid, dim3(kBlock), 0, st, n, ea, flag);
+    }
+    SPHK_TRY(hipGetLastError());
+    return SPHK_OK;
+}
 
 int sphk_copy_dev(void *dst, const void *src, size_t bytes, void *stream)
 {

@@ -117,8 +117,10 @@ _KDK_STAGES = (
                                                                    "room_copy")),
     ("scans and sorts (rocPRIM)", ("rocprim",)),
     ("tree rebuild (keys, levels, moments, node records, groups)",
-     ("split_kernel", "moments_kernel", "obox_kernel", "next_kernel", "pack_", "bbox_", "keys_kernel",
+     ("split_kernel", "moments_kernel", "obox_kernel", "pack_", "bbox_", "keys_kernel",
       "unsorted_kernel", "root_kernel", "group_", "head_flag", "fill_kernel")),
+    ("Morton reorder of the state, unchanged-input tests of the rebuilds",
+     ("permute_kernel", "equal_kernel", "flag_zero_kernel")),
     ("density functor", ("density_kernel",)),
     ("gravity functor", ("gravity_kernel",)),
     ("hydro functor", ("hydro_kernel",)),

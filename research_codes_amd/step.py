@@ -27,7 +27,7 @@ import torch
 from . import _lib as L
 from ._lib import check, lib
 from .device import FP64, MIXED, Context, Walks, to_device
-from .tree import DeviceTree, HostTree, morton_order, morton_order_dev
+from .tree import DeviceTree, HostTree, morton_order, morton_order_dev, permute_dev
 
 KSRH = 1.936492
 
@@ -154,6 +154,12 @@ class SPHSystem:
                 return
             pt = torch.from_numpy(perm.astype(np.int64)).to(self.dev)
         self.last_perm = pt.long()
+        if self.tree == "device":
+            # every state row and per-particle array in one gather and one copy back
+            cols = [self._sbuf[q] for q in range(self._sbuf.shape[0])]
+            cols += [self.id, self.istar, self.rs, self.np_, self.itr, self.eps2]
+            self._perm_work = permute_dev(pt.to(torch.int32), cols, getattr(self, "_perm_work", None))
+            return
         self._sbuf.copy_(self._sbuf[:, self.last_perm])
         for t in (self.id, self.istar, self.rs, self.np_, self.itr, self.eps2):
             t.copy_(t[pt])

@@ -146,6 +146,25 @@ def _copy_out(ptr, n: int, dtype):
     return out
 
 
+def permute_dev(perm, cols, work=None):
+    """Reorder CUDA tensors in place, col[k] = old col[perm[k]], with perm an
+    int32 permutation tensor (sphk_permute_dev: two launches for all columns).
+    `work` is reused scratch (a uint8 CUDA tensor, grown when too small);
+    returns the scratch for the next call."""
+    import torch
+    n, cnt = perm.numel(), len(cols)
+    if any(c.numel() != n for c in cols) or perm.dtype != torch.int32:
+        raise ValueError("permute_dev: columns of the permutation's length and an int32 permutation expected")
+    elem = (C.c_int * cnt)(*[c.element_size() for c in cols])
+    need = int(lib().sphk_permute_work_bytes(n, cnt, elem))
+    if work is None or work.numel() < need:
+        work = torch.empty(need, dtype=torch.uint8, device=perm.device)
+    ptrs = (_P * cnt)(*[_dptr(c) for c in cols])
+    check(lib().sphk_permute_dev(n, _dptr(perm), cnt, ptrs, elem, work.data_ptr(), work.numel(), _stream()),
+          "sphk_permute_dev")
+    return work
+
+
 def morton_order_dev(x, y, z):
     """Device permutation into Morton order (int64 tensor), as morton_order."""
     import torch
@@ -193,9 +212,17 @@ class DeviceTree:
         if last is not None and last[5:] == src[5:] and all(
                 (a is None) == (b is None) and (a is None or a.shape == b.shape)
                 for a, b in zip(last[:5], src[:5])):
-            # one device-side comparison of all inputs, one host read
-            same = torch.stack([(a == b).all() for a, b in zip(last[:5], src[:5]) if a is not None])
-            if bool(same.all()):
+            # one device-side bitwise comparison of all inputs, one host read
+            pairs = [(a, b) for a, b in zip(last[:5], src[:5]) if a is not None]
+            if any(b.dtype != torch.float64 for _, b in pairs):
+                raise ValueError("DeviceTree inputs must be float64")
+            if getattr(self, "_flag", None) is None:
+                self._flag = torch.zeros(1, dtype=torch.int32, device=x.device)
+            A = (_P * len(pairs))(*[a.data_ptr() for a, _ in pairs])
+            B = (_P * len(pairs))(*[_dptr(b) for _, b in pairs])
+            check(lib().sphk_equal_dev(x.numel(), len(pairs), A, B, self._flag.data_ptr(), _stream()),
+                  "sphk_equal_dev")
+            if int(self._flag.item()) == 0:
                 return self
         self.n = x.numel()
         self._src = None

@@ -195,9 +195,55 @@ def test_device_resort_matches_host():
             g = torch.Generator(device="cpu").manual_seed(3)
             shift = (torch.rand(s.n, generator=g, dtype=torch.float64) - 0.5).cuda() * 0.05
             s.s["x"].add_(shift)
+            s.rs.copy_(torch.arange(s.n, dtype=torch.float64, device="cuda"))
+            s.np_.copy_(torch.arange(s.n, device="cuda") * 3)
+            s.itr.copy_((torch.arange(s.n, device="cuda") % 7).to(torch.uint8))
+            s.eps2.copy_(s.s["y"] * 2.0)
             s.resort()
-            res.append((s.id.clone(), s.s["x"].clone()))
-    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
+            assert s.last_perm is not None
+            res.append([s.id, s.istar, s._sbuf, s.rs, s.np_, s.itr, s.eps2])
+    for a, b in zip(*res):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 257, 10001])
+def test_permute_and_equal_entry_points(n):
+    """sphk_permute_dev against torch indexing on columns of 1, 2, 4 and 8
+    bytes (rows of a 2D array with an odd length are only 8-byte aligned);
+    sphk_equal_dev is bitwise: -0.0 differs from 0.0, a NaN equals itself."""
+    from research_codes_amd.tree import permute_dev
+    g = torch.Generator(device="cpu").manual_seed(n)
+    perm = torch.randperm(n, generator=g).to(torch.int32).cuda()
+    rows = torch.rand(3, n, generator=g, dtype=torch.float64).cuda()
+    cols = [rows[0], rows[1], rows[2], torch.randint(0, 255, (n,), generator=g, dtype=torch.uint8).cuda(),
+            torch.randint(-9, 9, (n,), generator=g, dtype=torch.int16).cuda(),
+            torch.randint(-9, 9, (n,), generator=g, dtype=torch.int32).cuda(),
+            torch.randint(-9, 9, (n,), generator=g, dtype=torch.int64).cuda()]
+    want = [c[perm.long()].clone() for c in cols]
+    work = permute_dev(perm, cols)
+    work = permute_dev(torch.arange(n, dtype=torch.int32, device="cuda"), cols, work)   # identity, reused scratch
+    for c, w in zip(cols, want):
+        assert torch.equal(c, w)
+    a = torch.rand(n, generator=g, dtype=torch.float64).cuda()
+    flag = torch.ones(1, dtype=torch.int32, device="cuda")
+
+    def equal(x, y, count=2):
+        A = (L._P * count)(*[x.data_ptr()] * count)
+        B = (L._P * count)(*[y.data_ptr()] * count)
+        L.check(L.lib().sphk_equal_dev(n, count, A, B, flag.data_ptr(), torch.cuda.current_stream().cuda_stream),
+                "sphk_equal_dev")
+        return int(flag.item()) == 0
+
+    b = a.clone()
+    assert equal(a, b) and equal(a, b, 0)
+    b[-1] = float("nan")
+    a[-1] = float("nan")
+    assert equal(a, b)
+    a[-1], b[-1] = 0.0, -0.0
+    assert not equal(a, b)
+    b[-1] = 0.0
+    b[0] = torch.nextafter(b[0], torch.tensor(2.0, dtype=torch.float64, device="cuda"))
+    assert not equal(a, b)
 
 
 def test_short_lists_beyond_the_walk_room():
