@@ -133,7 +133,7 @@ def kdk_stages(trace_csv: str, last: int = 5) -> tuple[float, float, list[tuple[
     """Wall time and kernel time per KDK step over the last `last` steps
     (from the last `last` predict launches to the end of the trace), and per
     stage (calls per step, ms per step); kernels matching no stage are torch's
-    own (reorder gathers, comparisons, list copies)."""
+    own (comparisons, list copies)."""
     with open(trace_csv, newline="") as f:
         rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
     starts = [i for i, r in enumerate(rows) if "predict_kernel" in r["Kernel_Name"]]
@@ -145,7 +145,7 @@ def kdk_stages(trace_csv: str, last: int = 5) -> tuple[float, float, list[tuple[
     for r in seg:
         name = r["Kernel_Name"]
         stage = next((s for s, keys in _KDK_STAGES if any(k in name for k in keys)),
-                     "torch ops (reorder gathers, comparisons, list copies)")
+                     "torch ops (comparisons, list copies)")
         acc[stage] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
         cnt[stage] += 1
     out = sorted(((s, cnt[s] / last, acc[s] / last) for s in acc), key=lambda t: -t[2])
