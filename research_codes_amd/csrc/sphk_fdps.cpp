@@ -11,6 +11,7 @@
 // asynchronously.  Retrieve waits for
 // the result and scatters it back into FDPS's per-walk Force arrays in walk
 // order (as hdr_densityGPU.cu:594-621 does).
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -63,10 +64,15 @@ struct PinnedAlloc {
     T *allocate(size_t n)
     {
         void *p = nullptr;
-        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        if (n > SIZE_MAX / sizeof(T) ||
+            hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault) != hipSuccess)
+            throw std::bad_alloc();
         return static_cast<T *>(p);
     }
-    void deallocate(T *p, size_t) { (void)hipHostFree(p); }
+    void deallocate(T *p, size_t)
+    {
+        if (p) (void)hipHostFree(p);
+    }
 };
 template <typename T, typename U>
 bool operator==(const PinnedAlloc<T> &, const PinnedAlloc<U> &)
