@@ -951,6 +951,7 @@ struct sphk_dtree {
         for (auto *a : {&i64a, &i64b, &i64c, &i64d, &total}) a->release();
         err.release();
         scratch.release();
+        sscr.release();
         for (auto &L : lists) {
             L.j_off.release();
             L.j_idx.release();
