@@ -257,3 +257,8 @@ def test_list_cap_parameter_is_validated(ctx):
     import ctypes
     p = D.L.Params(1.6, 1.936492, 1e30, 0, 0, 2)      # persistent must be 0 or 1
     assert D.L.lib().sphk_set_params(ctx._h, ctypes.byref(p)) != 0
+    p = D.L.Params(1.6, 1.936492, 1e30, 0, 0, 0, 3)   # pinned_src must be 0 or 1
+    assert D.L.lib().sphk_set_params(ctx._h, ctypes.byref(p)) != 0
+    q = D.L.Params()
+    assert D.L.lib().sphk_get_params(ctx._h, ctypes.byref(q)) == 0
+    assert q.persistent == 0 and q.pinned_src == 0       # rejected values left the context as it was
