@@ -67,6 +67,17 @@ typedef struct {
  * device-resident step points sphk_dens_out at the state arrays. */
 int sphk_eos_balsara_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, void *stream);
 
+/* One round of the smoothing-length retry loop (calcDensityKernel,
+ * hdr_density.hpp:382-414), after a density pass into `round` with the
+ * gather lists of radius rs: each particle still retrying (rs != 0) whose
+ * round converged (itr == 0) takes the round's dens, divv, rotv, grdh, ksr
+ * and np (copyFromForce, hdr_density.hpp:24-31) and gets rs = 0 (its rs is
+ * kept in rs_conv unless rs_conv is NULL); one that did not retries with
+ * rs * expand (jump != 0: at least expand * ksr of the round).  *retry
+ * (device int32) = 1 if any particle retries, else 0.  Asynchronous. */
+int sphk_density_commit_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_dens_out round, int64_t *np, double *rs,
+                            double *rs_conv, double expand, int jump, int32_t *retry, void *stream);
+
 /* eps2 = (ksr / kappa)^2 when eps_const <= 0, else eps_const^2. */
 int sphk_softening_dev(sphk_ctx *ctx, int n, const double *ksr, double ksrh, double eps_const,
                        double *eps2, void *stream);

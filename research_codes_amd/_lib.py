@@ -101,6 +101,8 @@ SIGNATURES = {
     # sphk_step.h
     "sphk_eos_balsara_dev": (C.c_int, [_P, C.c_int, State, Phys, _P]),
     "sphk_softening_dev": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P, _P]),
+    "sphk_density_commit_dev": (C.c_int, [_P, C.c_int, State, DensOut, _P, _P, _P, C.c_double, C.c_int, _P,
+                                          _P]),
     "sphk_gravity_writeback_dev": (C.c_int, [_P, C.c_int, State, Phys, GravOut, _P]),
     "sphk_hydro_prep_dev": (C.c_int, [_P, C.c_int, State, _P, _P, _P, _P, _P]),
     "sphk_hydro_writeback_dev": (C.c_int, [_P, C.c_int, State, Phys, HydroOut, _P]),

@@ -233,6 +233,43 @@ using namespace sphk;
 namespace {
 
 // The caller's stream; NULL is the HIP null stream (see include/sphk.h).
+// One round of the smoothing-length retry loop (calcDensityKernel,
+// hdr_density.hpp:382-414) for particle i: a particle still retrying
+// (rs != 0) whose round converged (itr == 0) takes the round's results
+// (copyFromForce, hdr_density.hpp:24-31) and gets rs = 0 (its rs to rs_conv
+// when given); one that did not converge retries with rs * expand (with
+// jump: at least expand * ksr of the round); flag is set if any retries.
+__global__ __launch_bounds__(kBlock) void density_commit_kernel(int n, sphk_state s, sphk_dens_out r,
+                                                                int64_t *__restrict__ np, double *__restrict__ rs,
+                                                                double *__restrict__ rs_conv, double expand, int jump,
+                                                                int32_t *__restrict__ flag)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double rsi = rs[i];
+    if (rsi == 0.) return;   // converged in an earlier round
+    if (r.itr[i] == 0) {
+        s.dens[i] = r.dens[i];
+        s.divv[i] = r.divv[i];
+        s.rotv[i] = r.rotv[i];
+        s.grdh[i] = r.grdh[i];
+        s.ksr[i] = r.ksr[i];
+        np[i] = r.np[i];
+        if (rs_conv) rs_conv[i] = rsi;
+        rs[i] = 0.;
+        return;
+    }
+    double grown = rsi * expand;
+    if (jump) {
+        const double k = expand * r.ksr[i];
+        grown = grown > k ? grown : k;
+    }
+    rs[i] = grown;
+    *flag = 1;   // every writer stores 1
+}
+
+__global__ void zero_flag_kernel(int32_t *flag) { *flag = 0; }
+
 hipStream_t stream_of(sphk_ctx *, void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 
 int launch_check(const char *what)
@@ -390,6 +427,24 @@ int sphk_eos_balsara_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_phys p, void *
     if (rc || n == 0) return rc;
     hipLaunchKernelGGL(eos_balsara_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream_of(ctx, stream), n, s, p);
     return launch_check("eos_balsara");
+}
+
+int sphk_density_commit_dev(sphk_ctx *ctx, int n, sphk_state s, sphk_dens_out round, int64_t *np, double *rs,
+                            double *rs_conv, double expand, int jump, int32_t *retry, void *stream)
+{
+    int rc = pre(ctx, n, &s);
+    if (rc) return rc;
+    if (!retry) return set_error(SPHK_E_ARG, "density_commit: null retry flag");
+    hipStream_t st = stream_of(ctx, stream);
+    hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(1), 0, st, retry);
+    if (n > 0) {
+        if (!np || !rs || !round.dens || !round.divv || !round.rotv || !round.grdh || !round.ksr || !round.np ||
+            !round.itr)
+            return set_error(SPHK_E_ARG, "density_commit: null array");
+        hipLaunchKernelGGL(density_commit_kernel, dim3(blocks(n)), dim3(kBlock), 0, st, n, s, round, np, rs,
+                           rs_conv, expand, jump, retry);
+    }
+    return launch_check("density_commit");
 }
 
 int sphk_softening_dev(sphk_ctx *ctx, int n, const double *ksr, double ksrh, double eps_const,

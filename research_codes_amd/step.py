@@ -26,7 +26,7 @@ import torch
 
 from . import _lib as L
 from ._lib import check, lib
-from .device import FP64, MIXED, Context, Walks, to_device
+from .device import FP64, MIXED, Context, Walks, _mk, to_device
 from .tree import DeviceTree, HostTree, morton_order, morton_order_dev, permute_dev
 
 KSRH = 1.936492
@@ -357,24 +357,20 @@ class SPHSystem:
                for k in ("dens", "divv", "rotv", "grdh", "ksr")}
         tmp["np"] = torch.empty(n, dtype=torch.int64, device=self.dev)
         tmp["itr"] = torch.empty(n, dtype=torch.uint8, device=self.dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        tmp_out = _mk(L.DensOut, L.DENS_OUT, tmp)
         for rnd in range(1, max_rounds + 1):
             self.build_lists(("dens",))
             self.density(out=tmp)
-            active = self.rs != 0.0
-            newly = active & (tmp["itr"] == 0)
-            retry = active & ~newly
             if rounds_out is not None:
-                rounds_out.append((self.lists["dens"], newly.clone()))
-            for k in ("dens", "divv", "rotv", "grdh", "ksr"):
-                s[k].copy_(torch.where(newly, tmp[k], s[k]))
-            self.np_.copy_(torch.where(newly, tmp["np"], self.np_))
-            if rs_conv is not None:
-                rs_conv.copy_(torch.where(newly, self.rs, rs_conv))
-            grown = self.rs * expand
-            if jump:
-                grown = torch.maximum(grown, expand * tmp["ksr"])
-            self.rs.copy_(torch.where(retry, grown, torch.where(newly, torch.zeros_like(grown), self.rs)))
-            if not bool(retry.any()):
+                rounds_out.append((self.lists["dens"], (self.rs != 0.0) & (tmp["itr"] == 0)))
+            # the round's bookkeeping in one kernel, one host read
+            check(lib().sphk_density_commit_dev(self.ctx.handle, n, self._ss, tmp_out, self.np_.data_ptr(),
+                                                self.rs.data_ptr(),
+                                                None if rs_conv is None else rs_conv.data_ptr(), float(expand),
+                                                int(bool(jump)), flag.data_ptr(), self._st()),
+                  "sphk_density_commit_dev")
+            if int(flag.item()) == 0:
                 break
         else:
             raise RuntimeError(f"density did not converge in {max_rounds} rounds")
