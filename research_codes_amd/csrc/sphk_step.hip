@@ -50,17 +50,19 @@ __global__ __launch_bounds__(kBlock) void gravity_writeback_kernel(int n, sphk_s
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double kinv = 1. / p.ksrh;
-    const double h = s.ksr[i];
     s.g2x[i] = p.G * g.ax[i];
     s.g2y[i] = p.G * g.ay[i];
     s.g2z[i] = p.G * g.az[i];
     // hdr_gravity.hpp:110-111: eta is dPhi/dH through eps = H/kappa; with a
     // constant softening the potential does not depend on H and eta is 0
-    if (p.eps_const > 0.)
+    // (then nothing the density pass writes is read here)
+    if (p.eps_const > 0.) {
         s.eta[i] = 0.;
-    else
+    } else {
+        const double kinv = 1. / p.ksrh;
+        const double h = s.ksr[i];
         s.eta[i] = h * h * kinv * kinv * s.grdh[i] / (3. * s.dens[i]) * g.eta[i];
+    }
     s.pot[i] = p.G * g.pot[i];
 }
 

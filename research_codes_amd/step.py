@@ -448,10 +448,26 @@ class SPHSystem:
         return g
 
     def force(self, precision=None):
-        """One force evaluation on resident lists (no list rebuild, no retry)."""
-        self.density(precision=precision)
-        self.eos_balsara()
-        self.gravity(precision)
+        """One force evaluation on resident lists (no list rebuild, no retry).
+
+        Under a constant softening the gravity functor and its write-back
+        read nothing the density pass and the EOS write: they run on a second
+        stream beside them (the two kernels share the GPU where either leaves
+        it idle), joined before the hydro prep, which needs both."""
+        if not self.phys.eps_const > 0:
+            self.density(precision=precision)
+            self.eos_balsara()
+            self.gravity(precision)
+        else:
+            cur = torch.cuda.current_stream()
+            if getattr(self, "_side", None) is None:
+                self._side = torch.cuda.Stream(device=cur.device)
+            self._side.wait_stream(cur)
+            with torch.cuda.stream(self._side):
+                self.gravity(precision)
+            self.density(precision=precision)
+            self.eos_balsara()
+            cur.wait_stream(self._side)
         self.hydro(precision)
 
     # ------------------------------------------------------------ integrator
