@@ -180,9 +180,17 @@ def main() -> int:
             ev1.record()
             torch.cuda.synchronize()
             wall += time.perf_counter() - t
-        # all kernels of the step run on torch's current stream, so the
-        # torch events bracket the whole step
+        # the gravity pass may run on a second stream, joined back before the
+        # hydro pass: the torch events on the current stream bracket the step
         step_ms = ev0.elapsed_time(ev1)  # last step
+        # each functor's own duration (HIP events on its launch stream) from
+        # the same steps run in order on one stream: beside another kernel a
+        # launch takes longer than it needs the GPU for
+        ctx.reset_kernel_time()
+        for _ in range(steps):
+            restore()
+            sysm.force(prec, overlap=False)
+        torch.cuda.synchronize()
         kern = {}
         for k, which in (("density", 0), ("hydro", 1), ("gravity", 2)):
             ms, cnt = ctx.kernel_time(which)

@@ -169,7 +169,7 @@ class SPHSystem:
         if "grav" in kinds:
             self._softening()
         self._build_lists_segments(kinds)
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()   # this stream only: a gravity pass may run beside it
         self.list_build_s = time.perf_counter() - t0
 
     def _tree(self, a: int, b: int, g: int):
@@ -447,28 +447,35 @@ class SPHSystem:
         self.ctx.timing(timing)
         return g
 
-    def force(self, precision=None):
+    def force(self, precision=None, overlap: bool = True):
         """One force evaluation on resident lists (no list rebuild, no retry).
 
         Under a constant softening the gravity functor and its write-back
-        read nothing the density pass and the EOS write: they run on a second
-        stream beside them (the two kernels share the GPU where either leaves
-        it idle), joined before the hydro prep, which needs both."""
-        if not self.phys.eps_const > 0:
+        read nothing the density pass and the EOS write: with overlap they run
+        on a second stream beside them (the two kernels share the GPU where
+        either leaves it idle), joined before the hydro prep, which needs
+        both.  overlap=False runs the four in order on one stream (each
+        kernel's own duration is then measurable)."""
+        if not (overlap and self.phys.eps_const > 0):
             self.density(precision=precision)
             self.eos_balsara()
             self.gravity(precision)
         else:
             cur = torch.cuda.current_stream()
-            if getattr(self, "_side", None) is None:
-                self._side = torch.cuda.Stream(device=cur.device)
-            self._side.wait_stream(cur)
-            with torch.cuda.stream(self._side):
+            side = self._side_stream()
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
                 self.gravity(precision)
             self.density(precision=precision)
             self.eos_balsara()
-            cur.wait_stream(self._side)
+            cur.wait_stream(side)
         self.hydro(precision)
+
+    def _side_stream(self):
+        """A second stream (default priority) for the gravity functor."""
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream(device=torch.cuda.current_stream().device)
+        return self._side
 
     # ------------------------------------------------------------ integrator
     def diagnostics(self) -> dict:
@@ -541,8 +548,10 @@ class SPHSystem:
         rebuild per step.  dt overrides the power-of-two timestep rule (a
         fixed schedule, e.g. for finite-difference sensitivities).  `record`
         (a dict) receives the step's reorder, H_max, lists and particle ids
-        (in the reordered order).  Returns the
-        diagnostics before the step."""
+        (in the reordered order).  Returns the diagnostics before the step.
+        (Gravity on a second stream beside the density retry loop measured
+        slower: the list walks and the retry loop's host reads leave it little
+        idle GPU, and the walks slow down beside it.)"""
         d = self.diagnostics()
         self.dt = self.next_dt(d["dt_min"]) if dt is None else float(dt)
         dt = self.dt
