@@ -214,6 +214,8 @@ def main() -> int:
         torch.cuda.synchronize()
         g_ms += ev0.elapsed_time(ev1)
     res["ms_per_step_hipgraph"] = round(g_ms / args.steps, 4)
+    res["step_streams"] = ("gravity + write-back on a second stream beside density + EOS (constant softening), "
+                           "joined before hydro; kernel_ms from the same steps run in order on one stream")
     del graph
     if not args.no_fp64:
         log("timing fp64")
