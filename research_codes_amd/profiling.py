@@ -124,7 +124,8 @@ _KDK_STAGES = (
     ("density functor", ("density_kernel",)),
     ("gravity functor", ("gravity_kernel",)),
     ("hydro functor", ("hydro_kernel",)),
-    ("step glue (EOS, write-backs, predict/correct, softening, reductions)", ("sphk::",)),
+    ("step glue (EOS, write-backs, predict/correct, softening, reductions, retry-round bookkeeping)",
+     ("sphk::", "density_commit_kernel")),
     ("copies and fills", ("copyBuffer", "fillBuffer")),
 )
 
